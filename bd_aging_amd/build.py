@@ -1,0 +1,45 @@
+"""In-tree build of libbd_hip.so (hipcc, gfx950) and of the bd_aging_self driver.
+
+Flags: -ffp-contract=off / -fno-fast-math keep every a*b+c two roundings so
+the kernels evaluate the same IEEE operations as the FP64 oracle;
+-packed-fp32-ops is disabled for all device code.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = ["-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+          f"--offload-arch={ARCH}",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+LIB_SOURCES = ["bd_engine.hip"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build failed: {' '.join(cmd)}")
+
+
+def build_lib(verbose=True):
+    out = os.path.join(HERE, "libbd_hip.so")
+    srcs = [os.path.join(CSRC, s) for s in LIB_SOURCES]
+    _run([HIPCC] + COMMON + ["-fPIC", "-shared", "-o", out] + srcs, verbose)
+    return out
+
+
+def build_all(verbose=True):
+    build_lib(verbose)
+
+
+if __name__ == "__main__":
+    build_all()

@@ -1,0 +1,210 @@
+"""numpy-facing wrapper of the C ABI (include/bd_abi.h).
+
+Engine mirrors the reference driver's use of its kernels (main.cu:417-586):
+``cell_locate`` -> ``compute_forces`` -> ``step`` ...  Positions are N x 3 float64
+arrays, big species first (main.cu:387-393).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import abi
+from .abi import BDParams, BDGeometry, BDObs, check, load
+
+SPECIES_BIG = 0
+SPECIES_SML = 1
+
+
+def box_side(n_big, n_sml, sigma_big, sigma_sml, phi):
+    """Edge of the cubic box holding the two species at packing fraction phi."""
+    vol = math.pi / 6.0 * (n_big * sigma_big ** 3 + n_sml * sigma_sml ** 3)
+    return (vol / phi) ** (1.0 / 3.0)
+
+
+def make_params(n_big, n_sml, side=None, phi=None, sigma_big=1.4, sigma_sml=1.0, dt=1e-4,
+                potential=abi.BD_POT_WCA, ntags_big=8, ntags_sml=8,
+                temp_protocol=abi.BD_TEMP_LINEAR, T0=1.0, Tf=1.0, t0=0.0, tf=0.0, period=1.0,
+                kappa=0.0, seed=12345, first_step=0, rng_lanes=0):
+    if side is None:
+        if phi is None:
+            raise ValueError("give side or phi")
+        side = box_side(n_big, n_sml, sigma_big, sigma_sml, phi)
+    p = BDParams()
+    p.n_big, p.n_sml = int(n_big), int(n_sml)
+    p.potential = int(potential)
+    p.ntags_big, p.ntags_sml = int(ntags_big), int(ntags_sml)
+    p.temp_protocol = int(temp_protocol)
+    p.rng_lanes = int(rng_lanes)
+    p.side = float(side)
+    p.sigma_big, p.sigma_sml = float(sigma_big), float(sigma_sml)
+    p.dt, p.kappa = float(dt), float(kappa)
+    p.T0, p.Tf, p.t0, p.tf, p.period = float(T0), float(Tf), float(t0), float(tf), float(period)
+    p.seed = int(seed) & ((1 << 64) - 1)
+    p.first_step = int(first_step)
+    return p
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _uptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def geometry(params):
+    lib = load()
+    g = BDGeometry()
+    check(lib.bd_compute_geometry(C.byref(params), C.byref(g)))
+    return g
+
+
+def random_positions(n, side, seed):
+    """Uniform positions in [0, side)^3 from a rannew64 lane (host, C ABI)."""
+    lib = load()
+    out = np.zeros((n, 3), dtype=np.float64)
+    check(lib.bd_random_positions(int(n), float(side), int(seed) & ((1 << 64) - 1), _dptr(out)))
+    return out
+
+
+class Engine:
+    """One engine of R replicas on the GPU (R = 1: the plain single system)."""
+
+    def __init__(self, params):
+        if isinstance(params, BDParams):
+            params = [params]
+        self.lib = load()
+        self.R = len(params)
+        arr = (BDParams * self.R)(*params)
+        self.params = list(params)
+        h = C.c_void_p()
+        check(self.lib.bd_create(arr, self.R, C.byref(h)))
+        self.h = h
+        self.n = [p.n_big + p.n_sml for p in params]
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if self.h:
+            self.lib.bd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state -------------------------------------------------------------
+    def geometry(self, r=0):
+        g = BDGeometry()
+        check(self.lib.bd_get_geometry(self.h, r, C.byref(g)))
+        return g
+
+    def set_positions(self, xyz, r=0):
+        a = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n[r], 3)
+        check(self.lib.bd_set_positions(self.h, r, _dptr(a)))
+
+    def positions(self, r=0):
+        a = np.zeros((self.n[r], 3))
+        check(self.lib.bd_get_positions(self.h, r, _dptr(a)))
+        return a
+
+    def raw_positions(self, r=0):
+        a = np.zeros((self.n[r], 3))
+        check(self.lib.bd_get_raw_positions(self.h, r, _dptr(a)))
+        return a
+
+    def forces(self, r=0):
+        f = np.zeros((self.n[r], 3))
+        v = np.zeros(self.n[r])
+        u = np.zeros(self.n[r])
+        check(self.lib.bd_get_forces(self.h, r, _dptr(f), _dptr(v), _dptr(u)))
+        return f, v, u
+
+    def set_forces(self, f, r=0):
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(self.n[r], 3)
+        check(self.lib.bd_set_forces(self.h, r, _dptr(a)))
+
+    def mark_origin(self):
+        check(self.lib.bd_mark_origin(self.h))
+
+    def step_index(self):
+        v = C.c_int64()
+        check(self.lib.bd_get_step(self.h, C.byref(v)))
+        return v.value
+
+    # -- hot path ----------------------------------------------------------
+    def cell_locate(self):
+        check(self.lib.bd_cell_locate(self.h))
+
+    def cells(self, species, r=0):
+        g = self.geometry(r)
+        p = self.params[r]
+        n = g.ncell_big if species == SPECIES_BIG else g.ncell_sml
+        ntags = p.ntags_big if species == SPECIES_BIG else p.ntags_sml
+        nocup = np.zeros(n ** 3, dtype=np.int32)
+        cv = np.zeros(n ** 3 * ntags, dtype=np.int32)
+        check(self.lib.bd_get_cells(self.h, r, species, _iptr(nocup), _iptr(cv)))
+        return nocup, cv
+
+    def compute_forces(self):
+        check(self.lib.bd_compute_forces(self.h))
+
+    def neighbours(self, maxn=64, r=0):
+        counts = np.zeros(self.n[r], dtype=np.int32)
+        lst = np.zeros(self.n[r] * maxn, dtype=np.int32)
+        check(self.lib.bd_neighbours(self.h, r, maxn, _iptr(counts), _iptr(lst)))
+        return counts, lst.reshape(self.n[r], maxn)
+
+    def update(self):
+        check(self.lib.bd_update(self.h))
+
+    def step(self, nsteps=1):
+        check(self.lib.bd_step(self.h, int(nsteps)))
+
+    def kernel_times(self, nsteps):
+        """ms spent in (update, cells, forces) over nsteps eager steps."""
+        ms = np.zeros(3)
+        check(self.lib.bd_kernel_times(self.h, int(nsteps), _dptr(ms)))
+        return ms
+
+    def rng_lane(self, lane, r=0):
+        ring = np.zeros(250, dtype=np.uint64)
+        lcg = C.c_uint64()
+        draws = C.c_uint64()
+        check(self.lib.bd_get_rng_lane(self.h, r, lane, _uptr(ring), C.byref(lcg),
+                                       C.byref(draws)))
+        return ring, lcg.value, draws.value
+
+    # -- measurements ------------------------------------------------------
+    def gder_counts(self, nbins, rng, r=0):
+        c = np.zeros(4 * nbins, dtype=np.uint64)
+        check(self.lib.bd_gder(self.h, r, int(nbins), float(rng), _uptr(c)))
+        return c.reshape(4, nbins)
+
+    def observables(self, q, r=0):
+        o = BDObs()
+        check(self.lib.bd_observables(self.h, r, float(q), C.byref(o)))
+        return {k: getattr(o, k) for k, _ in BDObs._fields_}
+
+
+def rng_stream(seed, lanes, ndraws):
+    lib = load()
+    out = np.zeros(lanes * ndraws, dtype=np.uint64)
+    check(lib.bd_rng_stream(int(seed) & ((1 << 64) - 1), int(lanes), int(ndraws), _uptr(out)))
+    return out.reshape(lanes, ndraws)
+
+
+def lane_seed(seed, lane):
+    return load().bd_lane_seed(int(seed) & ((1 << 64) - 1), int(lane))

@@ -1,0 +1,178 @@
+"""Throughput of the Brownian-dynamics step on one MI355X.
+
+Workload (the aging-quench configuration): N = 32768 bidisperse WCA particles
+(16384 sigma=1.4 + 16384 sigma=1.0), phi = 0.58, box L = 38.1161, FP64,
+Ermak-McCammon dt = 1e-4, equilibrated at T = 1.0 during the warm-up steps
+and instantly quenched to T = 0.1 before the timed steps.  One step =
+update + cell binning + forces for every particle (main.cu:494-586).
+
+Also reported: the 256-replica uneven batch (N_r log-uniform in [1000, 16000],
+phi_r in [0.52, 0.60], quench T_r in [0.05, 0.5], distinct seeds) in one
+engine, and the oracle's single-core CPU rate on a bounded sample.
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from bd_aging_amd.engine import Engine, make_params, box_side  # noqa: E402
+from bd_aging_amd.init import relax_overlaps, relax_batch  # noqa: E402
+
+HBM_PEAK = 8.0e12          # B/s, MI355X
+B_FORCES = 72              # algorithmic bytes per particle, force pass
+B_STEP = 228               # algorithmic bytes per particle-step (bin 36 + forces 72 + update 120)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def batch_specs(R=256, seed=20251203):
+    rng = np.random.default_rng(seed)
+    n = np.rint(np.exp(rng.uniform(np.log(1000), np.log(16000), R))).astype(int)
+    phi = rng.uniform(0.52, 0.60, R)
+    T = rng.uniform(0.05, 0.5, R)
+    return n, phi, T
+
+
+def run_plain(steps, warmup):
+    nb = ns = 16384
+    side = box_side(nb, ns, 1.4, 1.0, 0.58)
+    log(f"[bench] relaxing N=32768 phi=0.58 L={side:.4f}")
+    x = relax_overlaps(nb, ns, side, seed=1)
+    p = make_params(nb, ns, side=side, dt=1e-4, T0=1.0, Tf=0.1, t0=0.0, tf=0.0,
+                    first_step=-warmup, seed=777)
+    eng = Engine(p)
+    eng.set_positions(x)
+    log(f"[bench] warmup {warmup} steps at T=1.0")
+    eng.step(warmup)
+    eng.mark_origin()
+    log(f"[bench] timed {steps} steps at T=0.1")
+    t0 = time.perf_counter()
+    eng.step(steps)
+    dt = time.perf_counter() - t0
+    ksteps = min(200, steps)
+    ms = eng.kernel_times(ksteps) / ksteps
+    obs = eng.observables(7.0)
+    eng.close()
+    return {"N": nb + ns, "side": side, "seconds": dt, "rate": (nb + ns) * steps / dt,
+            "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p}
+
+
+def run_batch(steps, warmup):
+    n, phi, T = batch_specs()
+    specs, params = [], []
+    for r in range(len(n)):
+        nb = int(n[r]) // 2
+        ns = int(n[r]) - nb
+        side = box_side(nb, ns, 1.4, 1.0, float(phi[r]))
+        specs.append((nb, ns, side, 1.4, 1.0, 5000 + r))
+        params.append(make_params(nb, ns, side=side, dt=1e-4, T0=1.0, Tf=float(T[r]), t0=0.0,
+                                  tf=0.0, first_step=-warmup, seed=90000 + r))
+    log(f"[bench] relaxing 256 replicas, sum N = {int(n.sum())}")
+    xs = relax_batch(specs)
+    eng = Engine(params)
+    for r, x in enumerate(xs):
+        eng.set_positions(x, r)
+    eng.step(warmup)
+    eng.mark_origin()
+    log(f"[bench] batch timed {steps} steps")
+    t0 = time.perf_counter()
+    eng.step(steps)
+    dt = time.perf_counter() - t0
+    ksteps = min(50, steps)
+    ms = eng.kernel_times(ksteps) / ksteps
+    eng.close()
+    tot = int(n.sum())
+    return {"replicas": len(n), "sum_N": tot, "seconds": dt, "rate": tot * steps / dt,
+            "kernel_ms": ms.tolist()}
+
+
+def cpu_baseline(plain, budget_s=10.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    o = orc.System(plain["params"])
+    o.set_positions(plain["x"])
+    o.cell_locate()
+    o.compute_forces()
+    k = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        o.step(1)
+        k += 1
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": plain["N"] * k / dt, "unit": "particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"FP64 oracle, N=32768 phi=0.58, {k} steps after relaxation "
+                      f"({dt:.1f} s, one thread)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: replicas shard across GPUs with no per-step "
+              "exchange, so N>1 is not run here", file=sys.stderr)
+        sys.exit(2)
+    plain = run_plain(a.steps, a.warmup)
+    log(f"[bench] plain: {plain['rate']:.4e} particle-steps/s, kernel ms {plain['kernel_ms']}")
+    batch = None
+    if not a.no_batch:
+        batch = run_batch(a.batch_steps, min(a.warmup, 500))
+        log(f"[bench] batch: {batch['rate']:.4e} particle-steps/s")
+    cpu = None if a.no_cpu else cpu_baseline(plain)
+    N = plain["N"]
+    f_ms = plain["kernel_ms"][2]
+    achieved = B_FORCES * N / (f_ms * 1e-3)
+    out = {
+        "metric": "particle-steps/s at N=32768 phi=0.58 (plain single system)",
+        "value": plain["rate"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": plain["seconds"] / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded random placement + overlap removal, then BD",
+        "config": {"workload": "aging quench N=32768 phi=0.58 WCA sigma 1.4/1.0, dt=1e-4, "
+                               "T 1.0 -> 0.1", "N": N, "side": round(plain["side"], 4),
+                   "potential": "wca", "cells": "23^3 big + 32^3 small, ntags 8"},
+        "roofline": {"bound": "hbm", "kernel": "k_forces", "achieved": achieved / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": None, "bytes_per_particle": B_FORCES,
+                     "avg_launch_ms": f_ms},
+        "kernel_ms": {"update": plain["kernel_ms"][0], "cells": plain["kernel_ms"][1],
+                      "forces": plain["kernel_ms"][2]},
+        "step_hbm_frac": B_STEP * plain["rate"] / HBM_PEAK,
+        "cpu_baseline": cpu,
+    }
+    if batch:
+        out["batch"] = {"value": batch["rate"], "unit": "particle-steps/s",
+                        "replicas": batch["replicas"], "sum_N": batch["sum_N"],
+                        "steps": a.batch_steps, "ratio_vs_plain": batch["rate"] / plain["rate"],
+                        "step_hbm_frac": B_STEP * batch["rate"] / HBM_PEAK,
+                        "kernel_ms": dict(zip(["update", "cells", "forces"], batch["kernel_ms"]))}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

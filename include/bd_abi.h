@@ -1,0 +1,177 @@
+/*
+ * bd_abi.h -- C ABI of the MI355X Brownian-dynamics engine (libbd_hip.so).
+ *
+ * Plain C: fixed-width integers, doubles, opaque handles, int status codes.
+ * No HIP, torch or C++ types cross this boundary.  Every call returns BD_OK
+ * (0) or one of the BD_E_* codes; bd_last_error() returns a text message for
+ * the last failing call on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   parametros (encabezados.h:40-75)           -> bd_params + bd_geometry
+ *   cell geometry (main.cu:259-298)            -> bd_compute_geometry
+ *   cudaMallocManaged/cudaFree (main.cu:303-321,991-1014) -> bd_create/bd_destroy
+ *   set_vec_*_dev + cell_locate_dev (cell_locate.cu:110-209, main.cu:536-543)
+ *                                              -> bd_cell_locate / bd_get_cells
+ *   get_forces_same_dev + get_forces_diff_dev (get_forces.cu:297-558,
+ *     main.cu:556-585)                         -> bd_compute_forces
+ *   setup_rng_kernel (update_ermak.cu:85-89)   -> seeding inside bd_create
+ *   update_ermak_dev (update_ermak.cu:91-160)  -> bd_update / bd_step
+ *   calculate_temp_linear/_sine (set_temp.cu:7-23, main.cu:520-524)
+ *                                              -> evaluated on the device per step
+ *   get_gder_dev + host block reduce (get_gder.cu:69-127, main.cu:719-768)
+ *                                              -> bd_gder
+ *   fself_isotropic, MSD, energy, pressure (fself.cu:3-21, main.cu:627-638,
+ *     776-806)                                 -> bd_observables
+ *
+ * Threading: a handle is not thread-safe; distinct handles are independent.
+ * Every handle owns one HIP stream and its device memory; calls on one handle
+ * are ordered on that stream.
+ */
+#ifndef BD_ABI_H
+#define BD_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference exit()s or fails silently instead) ---- */
+enum {
+  BD_OK = 0,
+  BD_E_ARG = 1,           /* invalid argument or geometry */
+  BD_E_HIP = 2,           /* HIP runtime error */
+  BD_E_CELL_OVERFLOW = 3, /* a cell holds >= ntags particles (main.cu:655-659) */
+  BD_E_NAN = 4,           /* a non-finite position appeared */
+  BD_E_STATE = 5,         /* call out of order */
+  BD_E_NOMEM = 6
+};
+
+/* pair potentials (potentials.cu:54-76); HARMONIC is the overlap-removal
+ * potential of the initial-configuration generator */
+enum { BD_POT_WCA = 0, BD_POT_WCA_MODIFIED = 1, BD_POT_HARMONIC = 2 };
+/* temperature protocols (set_temp.cu:7-23) */
+enum { BD_TEMP_LINEAR = 0, BD_TEMP_SINE = 1 };
+enum { BD_SPECIES_BIG = 0, BD_SPECIES_SML = 1 };
+
+/* One system (one replica).  FP64 re-expression of parametros
+ * (encabezados.h:40-75): the derived cell fields live in bd_geometry.
+ * Particles are stored big species first (main.cu:387-393). */
+typedef struct bd_params {
+  int32_t n_big;          /* ngrain_big */
+  int32_t n_sml;          /* ngrain_sml */
+  int32_t potential;      /* BD_POT_* */
+  int32_t ntags_big;      /* cell capacity, big grid (1..32) */
+  int32_t ntags_sml;      /* cell capacity, small grid (1..32) */
+  int32_t temp_protocol;  /* BD_TEMP_* */
+  int32_t rng_lanes;      /* rannew64 lanes; 0 = one lane per particle */
+  int32_t reserved;
+  double side;            /* box edge L */
+  double sigma_big;
+  double sigma_sml;
+  double dt;
+  double kappa;           /* stiffness of BD_POT_HARMONIC */
+  double T0, Tf, t0, tf;  /* temperature protocol (main.cu:98-104) */
+  double period;          /* sine protocol only */
+  uint64_t seed;          /* master seed of the rannew64 lanes */
+  int64_t first_step;     /* step index ni of the first update (-ntrans) */
+} bd_params;
+
+/* Cell geometry derived from bd_params (main.cu:259-298 with the cutoff of
+ * the selected potential in place of sigma). */
+typedef struct bd_geometry {
+  int32_t ncell_big, ncell_sml;
+  int32_t nrange_bb, nrange_bs, nrange_sb, nrange_ss;
+  double cell_side_big, cell_side_sml;
+  double rc_bb, rc_bs, rc_ss; /* pair cutoffs */
+} bd_geometry;
+
+/* Per-sample observables of one replica (fself.cu:3-21, main.cu:627-638,
+ * 776-806). */
+typedef struct bd_obs {
+  double fself_big;   /* <sin(q|dr|)/(q|dr|)> over the big species */
+  double msd_big, msd_sml;
+  double epot_big, epot_sml; /* sum(0.5*pot)/N per species */
+  double virial;      /* sum of per-particle virials */
+  double z;           /* 1 + virial/(3 N T) */
+  double temperature; /* T after the last update (pars.temp_set) */
+} bd_obs;
+
+typedef struct bd_sys bd_sys;
+
+/* ---- geometry and lifetime ---- */
+int bd_compute_geometry(const bd_params* p, bd_geometry* g);
+/* Create an engine holding n_replicas independent systems (1 = the plain
+ * single system).  Replica r uses params[r]. */
+int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out);
+int bd_destroy(bd_sys* s);
+int bd_num_replicas(const bd_sys* s, int32_t* out);
+int bd_get_geometry(const bd_sys* s, int32_t replica, bd_geometry* g);
+
+/* ---- state transfer (host buffers owned by the caller; xyz is N x 3) ---- */
+int bd_set_positions(bd_sys* s, int32_t replica, const double* xyz);
+int bd_get_positions(bd_sys* s, int32_t replica, double* xyz);
+int bd_get_raw_positions(bd_sys* s, int32_t replica, double* xyz);
+int bd_get_forces(bd_sys* s, int32_t replica, double* f, double* vir, double* pot);
+int bd_set_forces(bd_sys* s, int32_t replica, const double* f);
+/* raw := pos, ini := pos for every replica (main.cu:497-506, at ni == 0) */
+int bd_mark_origin(bd_sys* s);
+int bd_get_step(const bd_sys* s, int64_t* ni);
+
+/* ---- the hot path ---- */
+/* Bin every replica into its two cell grids (cell_locate.cu:155-209). */
+int bd_cell_locate(bd_sys* s);
+/* Export one grid in the reference "tags" layout: nocup[ncell^3] and
+ * cell_vec[ncell^3 * ntags] (species-local indices, -1 in empty slots). */
+int bd_get_cells(bd_sys* s, int32_t replica, int32_t species, int32_t* nocup, int32_t* cell_vec);
+/* Forces, virials and potentials from the current cells (get_forces.cu). */
+int bd_compute_forces(bd_sys* s);
+/* Neighbour lists in traversal order (test hook): counts[N], list[N*maxn]
+ * holding particle indices (big first). */
+int bd_neighbours(bd_sys* s, int32_t replica, int32_t maxn, int32_t* counts, int32_t* list);
+/* One Ermak-McCammon update of every replica at the current step with the
+ * protocol temperature; advances the rannew64 lanes and the step index. */
+int bd_update(bd_sys* s);
+/* nsteps full steps: update -> temperature -> cell locate -> forces
+ * (main.cu:494-586); replayed through a captured HIP graph. */
+int bd_step(bd_sys* s, int32_t nsteps);
+
+/* Measurement hook (no reference counterpart): runs nsteps steps with the
+ * kernels launched one by one on the engine's stream and HIP events around
+ * each; adds the elapsed milliseconds of the update, cells and forces kernels
+ * into ms[0], ms[1], ms[2]. */
+int bd_kernel_times(bd_sys* s, int32_t nsteps, double* ms);
+
+/* ---- rannew64 lanes ---- */
+/* State of one lane of a replica: ring[250] (slot order) and the LCG word;
+ * *draws = number of draws taken so far by every lane. */
+int bd_get_rng_lane(bd_sys* s, int32_t replica, int32_t lane, uint64_t* ring250,
+                    uint64_t* lcg, uint64_t* draws);
+/* Stand-alone stream test: `lanes` independent rannew64 streams with
+ * idum = bd_lane_seed(seed, lane); writes ndraws outputs of each lane,
+ * lane-major (out[lane*ndraws + k]). */
+int bd_rng_stream(uint64_t seed, int32_t lanes, int32_t ndraws, uint64_t* out);
+uint64_t bd_lane_seed(uint64_t seed, uint64_t lane);
+
+/* ---- initial configurations (no reference counterpart: run-fself.sh:131-138
+ * runs external set_init_config / eliminate_overlap binaries that the
+ * reference does not contain) ---- */
+/* Uniform positions in [0, side)^3 (N x 3) from one rannew64 stream with
+ * idum = bd_lane_seed(seed, 0); host only, no device work. */
+int bd_random_positions(int32_t n, double side, uint64_t seed, double* xyz);
+
+/* ---- measurements ---- */
+/* Ordered-pair distance histograms bb, bs, sb, ss (get_gder.cu:69-127),
+ * integer counts added into counts[4 * nbins]. */
+int bd_gder(bd_sys* s, int32_t replica, int32_t nbins, double range, uint64_t* counts);
+int bd_observables(bd_sys* s, int32_t replica, double q, bd_obs* out);
+
+/* ---- errors ---- */
+const char* bd_last_error(void);
+const char* bd_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BD_ABI_H */

@@ -1,0 +1,281 @@
+"""GPU parity: the HIP engine (through the C ABI) against the FP64 oracle on
+the same seeded inputs.
+
+Bar: bit-exact cell lists, neighbour lists, rannew64 streams and g(r) counts;
+forces within 1e-12 relative (the kernels use the oracle's summation order,
+so they are expected to be bit-exact); trajectories within 1e-9 over 100
+steps; reductions within 1e-12 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import orc
+from bd_aging_amd import abi
+from bd_aging_amd.abi import BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC, BD_TEMP_SINE
+from bd_aging_amd.engine import Engine, make_params, rng_stream, box_side
+from bd_aging_amd.init import relax_overlaps
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RTOL = 1e-12
+TRAJ_TOL = 1e-9
+
+
+@pytest.fixture(scope="module")
+def config1_positions():
+    """N=2000 (1000 big sigma=1.4 + 1000 small sigma=1.0) at phi=0.50, no overlaps."""
+    side = box_side(1000, 1000, 1.4, 1.0, 0.50)
+    x = relax_overlaps(1000, 1000, side, seed=2024)
+    return side, x
+
+
+def pair(p, x):
+    e = Engine(p)
+    e.set_positions(x)
+    o = orc.System(p)
+    o.set_positions(x)
+    return e, o
+
+
+def assert_forces_close(fe, fo):
+    scale = max(np.abs(fo).max(), 1e-300)
+    assert np.all(np.abs(fe - fo) <= FORCE_RTOL * np.maximum(np.abs(fo), scale * 1e-3))
+
+
+# ------------------------------------------------------------------ rng
+def test_rng_stream_bit_exact():
+    seed, lanes, nd = 987654321, 96, 700
+    got = rng_stream(seed, lanes, nd)
+    for l in range(0, lanes, 7):
+        ref = orc.Stream(orc.lib().orc_lane_seed(seed, l)).u64(nd)
+        assert np.array_equal(got[l], ref), l
+
+
+# ------------------------------------------------------------------ cells
+@pytest.mark.parametrize("pot", [BD_POT_WCA, BD_POT_WCA_MODIFIED])
+def test_cells_bit_exact(config1_positions, pot):
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, potential=pot)
+    e, o = pair(p, x)
+    e.cell_locate()
+    assert o.cell_locate() == 0
+    for sp in (0, 1):
+        ne, ce = e.cells(sp)
+        no, co = o.cells(sp)
+        assert np.array_equal(ne, no)
+        assert np.array_equal(ce, co)
+    e.close()
+
+
+def test_cells_edge_positions():
+    p = make_params(200, 200, phi=0.1)
+    rng = np.random.default_rng(1)
+    x = rng.random((400, 3)) * p.side
+    g = orc.geometry(p)[1]
+    x[0] = [p.side, 0.0, g.cell_side_big]
+    x[1] = [p.side, p.side, p.side]
+    x[250] = [g.cell_side_sml * 3, 0.0, p.side]
+    e, o = pair(p, x)
+    e.cell_locate()
+    o.cell_locate()
+    for sp in (0, 1):
+        assert all(np.array_equal(a, b) for a, b in zip(e.cells(sp), o.cells(sp)))
+    e.close()
+
+
+# ------------------------------------------------------------------ forces
+@pytest.mark.parametrize("pot", [BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC])
+def test_forces_and_neighbours(config1_positions, pot):
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, potential=pot, kappa=30.0)
+    xx = x.copy()
+    if pot == BD_POT_HARMONIC:  # make some overlaps so the harmonic force is non-zero
+        xx[:50] += 0.05
+        xx %= side
+    e, o = pair(p, xx)
+    e.cell_locate()
+    o.cell_locate()
+    e.compute_forces()
+    o.compute_forces()
+    fe, ve, ue = e.forces()
+    fo, vo, uo = o.forces()
+    assert np.abs(fo).max() > 0
+    assert_forces_close(fe, fo)
+    assert_forces_close(ve, vo)
+    assert_forces_close(ue, uo)
+    exact = np.array_equal(fe, fo) and np.array_equal(ve, vo) and np.array_equal(ue, uo)
+    print(f"forces bit-exact: {exact}")
+    ce, le = e.neighbours(64)
+    co, lo = o.neighbours(64)
+    assert np.array_equal(ce, co)
+    for m in range(len(x)):
+        assert np.array_equal(le[m, :ce[m]], lo[m, :co[m]])
+    e.close()
+
+
+# ------------------------------------------------------------------ update
+def test_one_update_bit_exact(config1_positions):
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, T0=1.0, Tf=1.0, seed=99)
+    e, o = pair(p, x)
+    e.cell_locate()
+    o.cell_locate()
+    e.compute_forces()
+    o.compute_forces()
+    e.update()
+    assert o.update() == 0
+    assert np.array_equal(e.positions(), o.positions())
+    assert np.array_equal(e.raw_positions(), o.raw_positions())
+    for lane in (0, 1, 777, 1999):
+        ring_e, lcg_e, draws = e.rng_lane(lane)
+        ring_o, lcg_o, npp, nq = o.rng_lane(lane)
+        assert draws == 4
+        assert np.array_equal(ring_e, ring_o) and lcg_e == lcg_o
+    e.close()
+
+
+@pytest.mark.parametrize("lanes", [0, 600])
+def test_trajectory_100_steps(config1_positions, lanes):
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, T0=1.0, Tf=1.0, seed=7, rng_lanes=lanes)
+    e, o = pair(p, x)
+    e.step(100)
+    assert o.cell_locate() == 0
+    o.compute_forces()
+    assert o.step(100) == 0
+    pe, po = e.positions(), o.positions()
+    d = np.abs(pe - po)
+    d = np.minimum(d, side - d)
+    assert d.max() <= TRAJ_TOL
+    assert np.abs(e.raw_positions() - o.raw_positions()).max() <= TRAJ_TOL
+    print(f"trajectory bit-exact: {np.array_equal(pe, po)}")
+    fe, _, _ = e.forces()
+    fo, _, _ = o.forces()
+    assert np.allclose(fe, fo, rtol=1e-9, atol=1e-9 * np.abs(fo).max())
+    assert e.step_index() == o.step_index() == 100
+    e.close()
+
+
+def test_quench_and_sine_protocols(config1_positions):
+    side, x = config1_positions
+    # instant quench at t0 = tf = 0 after a 20-step transient (first_step = -20)
+    p = make_params(1000, 1000, side=side, T0=1.0, Tf=0.1, t0=0.0, tf=0.0, first_step=-20,
+                    seed=5)
+    e, o = pair(p, x)
+    e.step(40)
+    o.cell_locate()
+    o.compute_forces()
+    o.step(40)
+    assert np.array_equal(e.positions(), o.positions())
+    ob_e, ob_o = e.observables(7.0), o.observables(7.0)
+    assert ob_e["temperature"] == ob_o["temperature"] == 0.1
+    e.close()
+    p = make_params(1000, 1000, side=side, T0=0.2, Tf=1.0, t0=0.0, tf=1.0, period=0.003,
+                    temp_protocol=BD_TEMP_SINE, seed=6)
+    e, o = pair(p, x)
+    e.step(30)
+    o.cell_locate()
+    o.compute_forces()
+    o.step(30)
+    assert np.array_equal(e.positions(), o.positions())
+    e.close()
+
+
+def test_graph_replay_equals_eager(config1_positions):
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, seed=11)
+    a = Engine(p)
+    a.set_positions(x)
+    a.step(48)  # three replays of the 16-step graph
+    b = Engine(p)
+    b.set_positions(x)
+    for _ in range(48):
+        b.step(1)
+    assert np.array_equal(a.positions(), b.positions())
+    a.close()
+    b.close()
+
+
+# ------------------------------------------------------------------ batch
+def test_batch_replicas_equal_single_runs():
+    specs = [(300, 340, 0.52, 0.5, 1), (700, 650, 0.55, 0.2, 2), (1000, 1000, 0.60, 0.05, 3)]
+    params, starts = [], []
+    for nb, ns, phi, T, seed in specs:
+        side = box_side(nb, ns, 1.4, 1.0, phi)
+        starts.append(relax_overlaps(nb, ns, side, seed=seed))
+        params.append(make_params(nb, ns, side=side, T0=1.0, Tf=T, t0=0.0, tf=0.0,
+                                  first_step=-5, seed=1000 + seed))
+    batch = Engine(params)
+    for r, x in enumerate(starts):
+        batch.set_positions(x, r)
+    batch.step(37)
+    for r, (p, x) in enumerate(zip(params, starts)):
+        single = Engine(p)
+        single.set_positions(x)
+        single.step(37)
+        assert np.array_equal(batch.positions(r), single.positions())
+        assert np.array_equal(batch.raw_positions(r), single.raw_positions())
+        single.close()
+    # and replica 1 against the oracle
+    o = orc.System(params[1])
+    o.set_positions(starts[1])
+    o.cell_locate()
+    o.compute_forces()
+    o.step(37)
+    d = np.abs(batch.positions(1) - o.positions())
+    assert np.minimum(d, params[1].side - d).max() <= TRAJ_TOL
+    batch.close()
+
+
+# ------------------------------------------------------------------ measurements
+def test_gder_counts_bit_exact(config1_positions):
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side)
+    e, o = pair(p, x)
+    for nbins, rng in [(200, side / 2), (128, 5.0), (400, 15.0)]:
+        ce = e.gder_counts(nbins, rng)
+        co = o.gder_counts(nbins, rng)
+        assert np.array_equal(ce, co), (nbins, rng)
+    e.close()
+
+
+def test_observables_close(config1_positions):
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, T0=0.8, Tf=0.8, seed=3)
+    e, o = pair(p, x)
+    e.step(25)
+    o.cell_locate()
+    o.compute_forces()
+    o.step(25)
+    for q in (7.0, 2 * math.pi):
+        a, b = e.observables(q), o.observables(q)
+        for k in a:
+            assert math.isclose(a[k], b[k], rel_tol=1e-12, abs_tol=1e-14), (k, a[k], b[k])
+    e.close()
+
+
+# ------------------------------------------------------------------ failures
+def test_cell_overflow_is_reported():
+    p = make_params(100, 100, phi=0.1, ntags_big=2)
+    rng = np.random.default_rng(2)
+    x = rng.random((200, 3)) * p.side
+    x[1:4] = x[0] + 0.01  # four big particles in one cell (capacity 2)
+    e = Engine(p)
+    e.set_positions(x)
+    with pytest.raises(abi.BDError) as err:
+        e.cell_locate()
+    assert err.value.code == abi.BD_E_CELL_OVERFLOW
+    e.close()
+
+
+def test_relaxed_configuration_has_no_overlaps(config1_positions):
+    side, x = config1_positions
+    d = x[:, None, :] - x[None, :, :]
+    d -= side * np.floor(d / side + 0.5)
+    r = np.sqrt((d ** 2).sum(-1))
+    np.fill_diagonal(r, np.inf)
+    big = np.arange(2000) < 1000
+    sig = np.where(big[:, None] & big[None, :], 1.4, np.where(~big[:, None] & ~big[None, :], 1.0, 1.2))
+    assert (r / sig).min() >= 1.0 - 1.5e-3
