@@ -20,6 +20,9 @@ BD_E_NOMEM = 6
 BD_POT_WCA = 0
 BD_POT_WCA_MODIFIED = 1
 BD_POT_HARMONIC = 2
+BD_ORDER_REFERENCE = 1
+BD_ORDER_SPLIT4 = 4
+BD_ORDER_SPLIT8 = 8
 BD_TEMP_LINEAR = 0
 BD_TEMP_SINE = 1
 
@@ -64,6 +67,7 @@ _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int32)
 _U = C.POINTER(C.c_uint64)
+_F = C.POINTER(C.c_float)
 SIGNATURES = {
     "bd_compute_geometry": (C.c_int, [C.POINTER(BDParams), C.POINTER(BDGeometry)]),
     "bd_create": (C.c_int, [C.POINTER(BDParams), C.c_int32, C.POINTER(C.c_void_p)]),
@@ -83,6 +87,11 @@ SIGNATURES = {
     "bd_neighbours": (C.c_int, [_P, C.c_int32, C.c_int32, _I, _I]),
     "bd_update": (C.c_int, [_P]),
     "bd_step": (C.c_int, [_P, C.c_int32]),
+    "bd_autotune": (C.c_int, [_P, C.c_int32, C.c_char_p, _I]),
+    "bd_set_force_order": (C.c_int, [_P, C.c_int32]),
+    "bd_get_force_order": (C.c_int, [_P, _I]),
+    "bd_set_brick": (C.c_int, [_P, C.c_int32]),
+    "bd_get_brick": (C.c_int, [_P, _I]),
     "bd_kernel_times": (C.c_int, [_P, C.c_int32, _D]),
     "bd_get_rng_lane": (C.c_int, [_P, C.c_int32, C.c_int32, _U, _U, _U]),
     "bd_rng_stream": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, _U]),
@@ -90,6 +99,29 @@ SIGNATURES = {
     "bd_random_positions": (C.c_int, [C.c_int32, C.c_double, C.c_uint64, _D]),
     "bd_gder": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, _U]),
     "bd_observables": (C.c_int, [_P, C.c_int32, C.c_double, C.POINTER(BDObs)]),
+    "bd_gder_all": (C.c_int, [_P, C.c_int32, C.c_double, _U]),
+    "bd_set_origin": (C.c_int, [_P, C.c_int32]),
+    "bd_fself_origins": (C.c_int, [_P, C.c_double, C.c_int32, _D]),
+    "bd_cv3d_create": (C.c_int, [C.c_int32, C.c_double, C.POINTER(C.c_void_p)]),
+    "bd_cv3d_destroy": (C.c_int, [_P]),
+    "bd_cv3d_add": (C.c_int, [_P, _P, C.c_int32, C.c_int32]),
+    "bd_cv3d_add_host": (C.c_int, [_P, _D, _D, C.c_int32, C.c_double]),
+    "bd_cv3d_solve": (C.c_int, [_P]),
+    "bd_cv3d_get": (C.c_int, [_P, _D]),
+    "bd_cv3d_get_normal": (C.c_int, [_P, _D, _D]),
+    "bd_cv3d_samples": (C.c_int, [_P, _I]),
+    "bd_mlp_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                C.POINTER(C.c_void_p)]),
+    "bd_mlp_destroy": (C.c_int, [_P]),
+    "bd_mlp_num_params": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "bd_mlp_get_params": (C.c_int, [_P, _F]),
+    "bd_mlp_set_params": (C.c_int, [_P, _F]),
+    "bd_mlp_forward": (C.c_int, [_P, _F, C.c_int32, _F]),
+    "bd_mlp_gradients": (C.c_int, [_P, _F, _F, C.c_int32, _F, _F]),
+    "bd_mlp_train_step": (C.c_int, [_P, _F, _F, C.c_int32, C.c_float, _F]),
+    "bd_mlp_set_data": (C.c_int, [_P, _F, _F, C.c_int32]),
+    "bd_mlp_train": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _F]),
+    "bd_mlp_predict": (C.c_int, [_P, _F, C.c_int32, _F, _F]),
     "bd_last_error": (C.c_char_p, []),
     "bd_strerror": (C.c_char_p, [C.c_int]),
 }

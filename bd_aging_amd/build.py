@@ -18,7 +18,7 @@ COMMON = ["-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
           f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["bd_engine.hip"]
+LIB_SOURCES = ["bd_engine.hip", "bd_cv3d.hip", "bd_mlp.hip"]
 
 
 def _run(cmd, verbose):
@@ -37,8 +37,18 @@ def build_lib(verbose=True):
     return out
 
 
+def build_driver(verbose=True):
+    """bd_aging_self: host C++ on the C ABI (no device code)."""
+    out = os.path.join(HERE, "bd_aging_self")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", out,
+          os.path.join(CSRC, "bd_aging_self.cpp"), "-L" + HERE, "-lbd_hip",
+          "-L/opt/rocm/lib", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"], verbose)
+    return out
+
+
 def build_all(verbose=True):
     build_lib(verbose)
+    build_driver(verbose)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,355 @@
+// bd_cv3d.hip -- force contraction (Contracting_Vector3D.cu:11-304) on gfx950.
+//
+// For every particle m, the unit vectors to its neighbours within `range`
+// are summed per distance bin into hit vectors h_m[bin] (x, y, z); the
+// normal equations A^T A (per axis, bins x bins) and A^T b (b = the
+// particle's force) are accumulated over particles, and calculate_vector
+// solves them with a truncated pseudo-inverse and averages the three axes.
+//
+// Kernels (FP64 VALU; the matrix cores are reserved for the surrogate):
+//   k_cv3d_hits  one thread per particle, candidates in ascending index
+//                order through LDS tiles; the particle's own hit row is
+//                accumulated in that order (Contracting_Vector3D.cu:85-131).
+//   k_cv3d_ata   SYRK-like 16x16 tiles of A^T A per axis, the sum over
+//                particles taken in ascending particle order so each entry
+//                is the same sequence of IEEE additions as the serial loop
+//                (Contracting_Vector3D.cu:135-148).
+//   k_cv3d_atb   one thread per (axis, bin), same order (:152-156).
+// The solve (bins^3) runs on the host: cyclic Jacobi eigen-decomposition of
+// the symmetric matrix, singular values |lambda| truncated at 1e-6 of the
+// largest (:194-282).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/bd_abi.h"
+#include "bd_internal.h"
+
+#pragma clang fp contract(off)
+
+struct bd_cv3d {
+  int bins = 0;
+  double range = 0.0;
+  int samples = 0;
+  bool solved = false;
+  double* ata = nullptr;  // 3 x bins x bins
+  double* atb = nullptr;  // 3 x bins
+  double* hit = nullptr;  // n x 3 x bins scratch
+  size_t hit_cap = 0;
+  double* in = nullptr;   // 6 x n staging of host inputs
+  size_t in_cap = 0;
+  hipStream_t stream = nullptr;
+  std::vector<double> result;
+};
+
+namespace {
+
+constexpr int kHitThreads = 256;
+constexpr int kTile = 16;
+
+#define CV_TRY(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return bdi::set_error(BD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+__global__ void __launch_bounds__(kHitThreads)
+    k_cv3d_hits(const double* __restrict__ px, const double* __restrict__ py,
+                const double* __restrict__ pz, int n, double side, int periodic,
+                double range_sq, double bin_size_inv, int bins, double* __restrict__ hit) {
+  __shared__ double tx[kHitThreads], ty[kHitThreads], tz[kHitThreads];
+  const int m = (int)(blockIdx.x * kHitThreads + threadIdx.x);
+  const bool active = m < n;
+  double xm = 0, ym = 0, zm = 0;
+  if (active) {
+    xm = px[m];
+    ym = py[m];
+    zm = pz[m];
+  }
+  const double side_inv = periodic ? 1.0 / side : 0.0;
+  double* row = hit + (size_t)(active ? m : 0) * 3 * bins;
+  for (int jt = 0; jt < n; jt += kHitThreads) {
+    const int jl = jt + (int)threadIdx.x;
+    if (jl < n) {
+      tx[threadIdx.x] = px[jl];
+      ty[threadIdx.x] = py[jl];
+      tz[threadIdx.x] = pz[jl];
+    }
+    __syncthreads();
+    const int tn = min(kHitThreads, n - jt);
+    if (active) {
+      for (int k = 0; k < tn; k++) {
+        if (jt + k == m) continue;
+        double dx = xm - tx[k];
+        double dy = ym - ty[k];
+        double dz = zm - tz[k];
+        if (periodic) {
+          dx -= side * floor(side_inv * dx + 0.5);
+          dy -= side * floor(side_inv * dy + 0.5);
+          dz -= side * floor(side_inv * dz + 0.5);
+        }
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < range_sq) {
+          const double dist = sqrt(d2);
+          int nb = (int)(bin_size_inv * dist);
+          if (nb >= bins) nb = bins - 1;
+          row[nb] += dx / dist;
+          row[bins + nb] += dy / dist;
+          row[2 * bins + nb] += dz / dist;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One 16x16 tile (i-tile >= j-tile) of A^T A for axis blockIdx.z.
+__global__ void __launch_bounds__(kTile * kTile)
+    k_cv3d_ata(const double* __restrict__ hit, int n, int bins, double* __restrict__ ata) {
+  const int ti = (int)blockIdx.y, tj = (int)blockIdx.x;
+  if (tj > ti) return;
+  const int a = (int)blockIdx.z;
+  __shared__ double hi[kTile][kTile + 1], hj[kTile][kTile + 1];
+  const int li = (int)threadIdx.x / kTile, lj = (int)threadIdx.x % kTile;
+  const int i = ti * kTile + li, j = tj * kTile + lj;
+  const bool live = i < bins && j < bins && j <= i;
+  double* A = ata + (size_t)a * bins * bins;
+  double acc = live ? A[(size_t)i * bins + j] : 0.0;
+  for (int m0 = 0; m0 < n; m0 += kTile) {
+    // load rows m0..m0+15 of the i-tile and j-tile columns
+    const int mm = li, col = lj;
+    const int m = m0 + mm;
+    hi[mm][col] = (m < n && ti * kTile + col < bins)
+                      ? hit[((size_t)m * 3 + a) * bins + ti * kTile + col] : 0.0;
+    hj[mm][col] = (m < n && tj * kTile + col < bins)
+                      ? hit[((size_t)m * 3 + a) * bins + tj * kTile + col] : 0.0;
+    __syncthreads();
+    const int mend = min(kTile, n - m0);
+    if (live)
+      for (int k = 0; k < mend; k++) {
+        const double aux = hi[k][li] * hj[k][lj];
+        acc += aux;
+      }
+    __syncthreads();
+  }
+  if (live) {
+    A[(size_t)i * bins + j] = acc;
+    A[(size_t)j * bins + i] = acc;
+  }
+}
+
+__global__ void k_cv3d_atb(const double* __restrict__ hit, const double* __restrict__ fx,
+                           const double* __restrict__ fy, const double* __restrict__ fz, int n,
+                           int bins, double* __restrict__ atb) {
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= 3 * bins) return;
+  const int a = k / bins, i = k % bins;
+  const double* f = a == 0 ? fx : (a == 1 ? fy : fz);
+  double acc = atb[k];
+  for (int m = 0; m < n; m++) acc += hit[((size_t)m * 3 + a) * bins + i] * f[m];
+  atb[k] = acc;
+}
+
+int accumulate(bd_cv3d* cv, const double* px, const double* py, const double* pz,
+               const double* fx, const double* fy, const double* fz, int n, double side,
+               int periodic, hipStream_t st) {
+  const size_t need = (size_t)n * 3 * cv->bins;
+  if (need > cv->hit_cap) {
+    if (cv->hit) (void)hipFree(cv->hit);
+    cv->hit = nullptr;
+    cv->hit_cap = 0;
+    CV_TRY(hipMalloc((void**)&cv->hit, need * sizeof(double)));
+    cv->hit_cap = need;
+  }
+  CV_TRY(hipMemsetAsync(cv->hit, 0, need * sizeof(double), st));
+  const double bin_size = cv->range / (double)cv->bins;
+  const double bin_size_inv = 1.0 / bin_size;
+  k_cv3d_hits<<<(n + kHitThreads - 1) / kHitThreads, kHitThreads, 0, st>>>(
+      px, py, pz, n, side, periodic, cv->range * cv->range, bin_size_inv, cv->bins, cv->hit);
+  CV_TRY(hipGetLastError());
+  const int nt = (cv->bins + kTile - 1) / kTile;
+  k_cv3d_ata<<<dim3(nt, nt, 3), kTile * kTile, 0, st>>>(cv->hit, n, cv->bins, cv->ata);
+  CV_TRY(hipGetLastError());
+  k_cv3d_atb<<<(3 * cv->bins + 255) / 256, 256, 0, st>>>(cv->hit, fx, fy, fz, n, cv->bins,
+                                                          cv->atb);
+  CV_TRY(hipGetLastError());
+  CV_TRY(hipStreamSynchronize(st));
+  cv->samples++;
+  cv->solved = false;
+  return BD_OK;
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric matrix (Rutishauser's
+// rotation, threshold sweeps): a (destroyed) -> eigenvalues w, vectors v
+// (columns).
+void jacobi_eigen(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& v) {
+  v.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.0;
+  auto A = [&](int i, int j) -> double& { return a[(size_t)i * n + j]; };
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        const double q = A(i, j) * A(i, j);
+        tot += q;
+        if (i != j) off += q;
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = A(p, q);
+        if (apq == 0.0) continue;
+        const double theta = (A(q, q) - A(p, p)) / (2.0 * apq);
+        const double t = std::copysign(1.0, theta) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; k++) {
+          const double akp = A(k, p), akq = A(k, q);
+          A(k, p) = c * akp - s * akq;
+          A(k, q) = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          const double apk = A(p, k), aqk = A(q, k);
+          A(p, k) = c * apk - s * aqk;
+          A(q, k) = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          double* vr = &v[(size_t)k * n];
+          const double vkp = vr[p], vkq = vr[q];
+          vr[p] = c * vkp - s * vkq;
+          vr[q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; i++) w[i] = A(i, i);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bd_cv3d_create(int32_t bins, double range, bd_cv3d** out) {
+  if (!out || bins < 1 || bins > 4096 || !(range > 0.0))
+    return bdi::set_error(BD_E_ARG, "bins must be in 1..4096 and range > 0");
+  bd_cv3d* cv = new bd_cv3d();
+  cv->bins = bins;
+  cv->range = range;
+  const size_t bb = (size_t)bins * bins;
+  hipError_t e = hipMalloc((void**)&cv->ata, 3 * bb * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&cv->atb, 3 * (size_t)bins * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(cv->ata, 0, 3 * bb * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(cv->atb, 0, 3 * (size_t)bins * sizeof(double));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&cv->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    if (cv->ata) (void)hipFree(cv->ata);
+    if (cv->atb) (void)hipFree(cv->atb);
+    delete cv;
+    return bdi::set_error(BD_E_HIP, std::string("bd_cv3d_create: ") + hipGetErrorString(e));
+  }
+  *out = cv;
+  return BD_OK;
+}
+
+int bd_cv3d_destroy(bd_cv3d* cv) {
+  if (!cv) return BD_OK;
+  if (cv->stream) (void)hipStreamSynchronize(cv->stream);
+  double* p[] = {cv->ata, cv->atb, cv->hit, cv->in};
+  for (double* q : p)
+    if (q) (void)hipFree(q);
+  if (cv->stream) (void)hipStreamDestroy(cv->stream);
+  delete cv;
+  return BD_OK;
+}
+
+int bd_cv3d_add(bd_cv3d* cv, bd_sys* s, int32_t replica, int32_t periodic) {
+  if (!cv) return bdi::set_error(BD_E_ARG, "null contraction handle");
+  bdi::DevState d;
+  int rc = bdi::device_state(s, replica, &d);
+  if (rc != BD_OK) return rc;
+  CV_TRY(hipStreamSynchronize(d.stream));  // positions and forces of the last step
+  return accumulate(cv, d.px, d.py, d.pz, d.fx, d.fy, d.fz, d.n, d.side, periodic ? 1 : 0,
+                    cv->stream);
+}
+
+int bd_cv3d_add_host(bd_cv3d* cv, const double* xyz, const double* f, int32_t n, double side) {
+  if (!cv || !xyz || !f || n < 1) return bdi::set_error(BD_E_ARG, "bad arguments");
+  const size_t need = (size_t)6 * n;
+  if (need > cv->in_cap) {
+    if (cv->in) (void)hipFree(cv->in);
+    cv->in = nullptr;
+    cv->in_cap = 0;
+    CV_TRY(hipMalloc((void**)&cv->in, need * sizeof(double)));
+    cv->in_cap = need;
+  }
+  std::vector<double> h(need);
+  for (int i = 0; i < n; i++)
+    for (int a = 0; a < 3; a++) {
+      h[(size_t)a * n + i] = xyz[3 * i + a];
+      h[(size_t)(3 + a) * n + i] = f[3 * i + a];
+    }
+  CV_TRY(hipMemcpyAsync(cv->in, h.data(), need * sizeof(double), hipMemcpyHostToDevice,
+                        cv->stream));
+  const double* b = cv->in;
+  return accumulate(cv, b, b + n, b + 2 * (size_t)n, b + 3 * (size_t)n, b + 4 * (size_t)n,
+                    b + 5 * (size_t)n, n, side, side > 0.0 ? 1 : 0, cv->stream);
+}
+
+int bd_cv3d_get_normal(bd_cv3d* cv, double* ata, double* atb) {
+  if (!cv) return bdi::set_error(BD_E_ARG, "null contraction handle");
+  const size_t bb = (size_t)cv->bins * cv->bins;
+  if (ata) CV_TRY(hipMemcpy(ata, cv->ata, 3 * bb * sizeof(double), hipMemcpyDeviceToHost));
+  if (atb) CV_TRY(hipMemcpy(atb, cv->atb, 3 * (size_t)cv->bins * sizeof(double),
+                            hipMemcpyDeviceToHost));
+  return BD_OK;
+}
+
+int bd_cv3d_samples(const bd_cv3d* cv, int32_t* out) {
+  if (!cv || !out) return bdi::set_error(BD_E_ARG, "null argument");
+  *out = cv->samples;
+  return BD_OK;
+}
+
+// calculate_vector (Contracting_Vector3D.cu:166-297).  Unlike the reference,
+// a second call recomputes rather than adds to the previous result.
+int bd_cv3d_solve(bd_cv3d* cv) {
+  if (!cv) return bdi::set_error(BD_E_ARG, "null contraction handle");
+  if (cv->samples <= 0)
+    return bdi::set_error(BD_E_STATE, "Number of samples taken must be greater than zero.");
+  const int n = cv->bins;
+  const size_t bb = (size_t)n * n;
+  std::vector<double> ata(3 * bb), atb(3 * (size_t)n);
+  int rc = bd_cv3d_get_normal(cv, ata.data(), atb.data());
+  if (rc != BD_OK) return rc;
+  cv->result.assign(n, 0.0);
+  std::vector<double> a, w, v;
+  for (int ax = 0; ax < 3; ax++) {
+    a.assign(ata.begin() + ax * bb, ata.begin() + (ax + 1) * bb);
+    jacobi_eigen(n, a, w, v);
+    double wmax = 0.0;
+    for (int i = 0; i < n; i++) wmax = std::max(wmax, std::fabs(w[i]));
+    const double* b = &atb[(size_t)ax * n];
+    for (int k = 0; k < n; k++) {
+      if (!(wmax > 0.0) || !(std::fabs(w[k] / wmax) > 1.0e-6)) continue;
+      double proj = 0.0;
+      for (int i = 0; i < n; i++) proj += v[(size_t)i * n + k] * b[i];
+      const double c = proj / w[k];
+      for (int i = 0; i < n; i++) cv->result[i] += v[(size_t)i * n + k] * c;
+    }
+  }
+  for (int i = 0; i < n; i++) cv->result[i] /= 3.0;
+  cv->solved = true;
+  return BD_OK;
+}
+
+int bd_cv3d_get(const bd_cv3d* cv, double* out) {
+  if (!cv || !out) return bdi::set_error(BD_E_ARG, "null argument");
+  if (!cv->solved)
+    return bdi::set_error(BD_E_STATE, "Retrieved contracted vector must be firstly calculated.");
+  for (int i = 0; i < cv->bins; i++) out[i] = cv->result[i];
+  return BD_OK;
+}
+
+}  // extern "C"

@@ -41,6 +41,7 @@
 
 #include "../../include/bd_abi.h"
 #include "bd_device.h"
+#include "bd_internal.h"
 
 #pragma clang fp contract(off)
 
@@ -49,6 +50,7 @@ namespace {
 constexpr int kBlk = 256;
 constexpr int kFlagOverflow = 1;
 constexpr int kFlagNonFinite = 2;
+constexpr int kFlagBrick = 4;
 
 struct RepDev {
   int32_t n, nb;
@@ -61,8 +63,11 @@ struct RepDev {
   int32_t nrange_same[2];   // bb, ss
   int32_t nrange_cross[2];  // big particle -> small grid (bs), small -> big (sb)
   int32_t potential, temp_protocol;
+  int32_t fast;  // both grids >= 2*nrange+3 cells and nrange <= 2: scan_fast
+  int32_t pad_;
   double side, side_inv;
   double cell_inv[2];
+  double cell_side[2];
   double sigma[2];
   double sigma_bs;
   double rc2_same[2];
@@ -287,8 +292,25 @@ __global__ void __launch_bounds__(kBlk)
   if (n >= ntags) atomicOr(&flags[t.x], kFlagOverflow);
 }
 
+// Distance along one axis from a point to the cell at offset d of its home
+// cell (u = coordinate within the home cell), over the three periodic images.
+__device__ inline double axis_gap(int d, double cs, double u, double L) {
+  const double lo = (double)d * cs - u;
+  const double hi = lo + cs;
+  double g = lo > 0.0 ? lo : (hi < 0.0 ? -hi : 0.0);
+  const double lo2 = lo - L, hi2 = hi - L;
+  const double g2 = lo2 > 0.0 ? lo2 : (hi2 < 0.0 ? -hi2 : 0.0);
+  const double lo3 = lo + L, hi3 = hi + L;
+  const double g3 = lo3 > 0.0 ? lo3 : (hi3 < 0.0 ? -hi3 : 0.0);
+  g = g2 < g ? g2 : g;
+  return g3 < g ? g3 : g;
+}
+
 // Scan one grid around position (x, y, z) in the reference's order
 // (get_forces.cu:360-416 / 495-550): i_del, j_del, k_del nested, slot order.
+// A stencil cell whose nearest point is farther than the cutoff (plus a
+// 1e-9 margin that covers rounding) cannot hold an interacting partner and
+// is skipped; the interacting pairs and their order are unchanged.
 template <int POT, bool NEIGH>
 __device__ inline void scan_grid(const RepDev& R, int g, int nrange, double sigma, double rc2,
                                  int self, double x, double y, double z,
@@ -300,15 +322,26 @@ __device__ inline void scan_grid(const RepDev& R, int g, int nrange, double sigm
   const int n = R.ncell[g];
   const int ntags = R.ntags[g];
   const double side = R.side, side_inv = R.side_inv;
+  const double cs = R.cell_side[g];
+  const double ux = x - (double)ii * cs, uy = y - (double)jj * cs, uz = z - (double)kk * cs;
+  const double lim = rc2 + 1e-9;
   const int* cnt_g = cnt + R.cell_off[g];
   const double4* cp_g = cpos + R.tag_off[g];
   for (int di = -nrange; di <= nrange; di++) {
+    const double gx = axis_gap(di, cs, ux, side);
+    const double gx2 = gx * gx;
+    if (gx2 > lim) continue;
     int a = ii + di;
     a = a < 0 ? a + n : (a >= n ? a - n : a);
     for (int dj = -nrange; dj <= nrange; dj++) {
+      const double gy = axis_gap(dj, cs, uy, side);
+      const double gxy2 = gx2 + gy * gy;
+      if (gxy2 > lim) continue;
       int b = jj + dj;
       b = b < 0 ? b + n : (b >= n ? b - n : b);
       for (int dk = -nrange; dk <= nrange; dk++) {
+        const double gz = axis_gap(dk, cs, uz, side);
+        if (gxy2 + gz * gz > lim) continue;
         int c = kk + dk;
         c = c < 0 ? c + n : (c >= n ? c - n : c);
         const int ci = a + n * (b + n * c);
@@ -347,8 +380,195 @@ __device__ inline void scan_grid(const RepDev& R, int g, int nrange, double sigm
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fast stencil walk.
+//
+// Same pairs, same order and same IEEE operations as scan_grid, with less
+// bookkeeping:
+//  * the distance from the particle to a stencil slab along one axis is
+//    d*cs - u (d > 0), u + (|d|-1)*cs (d < 0) or 0, u the coordinate within
+//    the home cell; the innermost offsets' squared gaps are precomputed;
+//  * the minimum image: for a candidate in the cell at unwrapped coordinate
+//    h+d, floor(dx/L + 0.5) equals -K with K = -1 (h+d < 0), +1 (h+d >= n) or
+//    0, because |dx + K L| <= (NR+1) cs <= L/2 - cs/2 when n >= 2 NR + 3; so
+//    dx - L*floor(...) is exactly dx + K*L (and dx itself when K == 0).
+// Replicas whose grids are narrower than 2 NR + 3 cells (or NR > 2) use
+// scan_grid.  ACC maps accessor-frame cell coordinates to a slot range.
+// ---------------------------------------------------------------------------
+struct GlobalCells {
+  const int* cnt;
+  const double4* cp;
+  int n, ntags;
+  __device__ int index(int a, int b, int c) const {
+    a = a < 0 ? a + n : (a >= n ? a - n : a);
+    b = b < 0 ? b + n : (b >= n ? b - n : b);
+    c = c < 0 ? c + n : (c >= n ? c - n : c);
+    return a + n * (b + n * c);
+  }
+  __device__ void cell(int idx, int& occ, const double4*& ptr) const {
+    const int o = cnt[idx];
+    occ = o < ntags ? o : ntags;
+    ptr = cp + (size_t)idx * ntags;
+  }
+};
+
+struct LdsCells {
+  const unsigned int* tbl;  // (start << 6) | occupancy
+  const double4* spos;
+  int lo0, lo1, lo2, d0, d1;
+  __device__ int index(int a, int b, int c) const {
+    return (a - lo0) + d0 * ((b - lo1) + d1 * (c - lo2));
+  }
+  __device__ void cell(int idx, int& occ, const double4*& ptr) const {
+    const unsigned int e = tbl[idx];
+    occ = (int)(e & 63u);
+    ptr = spos + (e >> 6);
+  }
+};
+
+__device__ inline double slab_gap(int d, double cs, double u) {
+  return d > 0 ? (double)d * cs - u : (d < 0 ? u + (double)(-d - 1) * cs : 0.0);
+}
+
+__device__ inline double image_shift(int hd, int n, double L) {
+  return hd < 0 ? -L : (hd >= n ? L : 0.0);
+}
+
+// Walk one grid's stencil (nrange NR) around (x, y, z).  hw: home cell
+// (wrapped), hu: the same cell in the accessor's frame.  The innermost (dk)
+// row is unrolled: its occupancies and first slots are loaded together
+// before the pairs are visited in order.  SPLIT: the lane walks only the
+// cells whose running index cbase + flat is congruent to `sub` modulo G.
+template <int POT, int NR, bool SPLIT, class ACC>
+__device__ inline void scan_fast(const RepDev& R, int n, double cs, double sigma, double rc2,
+                                 int self, double x, double y, double z, const int hw[3],
+                                 const int hu[3], const ACC& acc, double& fx, double& fy,
+                                 double& fz, double& vir, double& pot, int sub, int G,
+                                 int cbase) {
+  const double L = R.side;
+  const double lim = rc2 + 1e-9;
+  const double ux = x - (double)hw[0] * cs, uy = y - (double)hw[1] * cs,
+               uz = z - (double)hw[2] * cs;
+  double g2z[2 * NR + 1], shz[2 * NR + 1];
+#pragma unroll
+  for (int k = 0; k <= 2 * NR; k++) {
+    const double gz = slab_gap(k - NR, cs, uz);
+    g2z[k] = gz * gz;
+    shz[k] = image_shift(hw[2] + k - NR, n, L);
+  }
+  for (int di = -NR; di <= NR; di++) {
+    const double gx = slab_gap(di, cs, ux);
+    const double gx2 = gx * gx;
+    if (gx2 > lim) continue;
+    const double shx = image_shift(hw[0] + di, n, L);
+    for (int dj = -NR; dj <= NR; dj++) {
+      const double gy = slab_gap(dj, cs, uy);
+      const double gxy2 = gx2 + gy * gy;
+      if (gxy2 > lim) continue;
+      const double shy = image_shift(hw[1] + dj, n, L);
+      const int rowflat = cbase + ((di + NR) * (2 * NR + 1) + (dj + NR)) * (2 * NR + 1);
+      int occk[2 * NR + 1];
+      const double4* cellk[2 * NR + 1];
+#pragma unroll
+      for (int k = 0; k <= 2 * NR; k++) {
+        occk[k] = 0;
+        cellk[k] = nullptr;
+        const bool mine = !SPLIT || (rowflat + k) % G == sub;
+        if (mine && gxy2 + g2z[k] <= lim)
+          acc.cell(acc.index(hu[0] + di, hu[1] + dj, hu[2] + k - NR), occk[k], cellk[k]);
+      }
+      double4 first[2 * NR + 1];
+#pragma unroll
+      for (int k = 0; k <= 2 * NR; k++)
+        if (occk[k] > 0) first[k] = cellk[k][0];
+#pragma unroll
+      for (int k = 0; k <= 2 * NR; k++) {
+        const int occ = occk[k];
+        if (occ == 0) continue;
+        const int dk = k - NR;
+        const double shzk = shz[k];
+        const double4* cell = cellk[k];
+        const bool center = self >= 0 && di == 0 && dj == 0 && dk == 0;
+        for (int s = 0; s < occ; s++) {
+          const double4 q = s == 0 ? first[k] : cell[s];
+          if (center && q.w == (double)self) continue;
+          double dx = q.x - x;
+          double dy = q.y - y;
+          double dz = q.z - z;
+          if (shx != 0.0) dx = dx + shx;
+          if (shy != 0.0) dy = dy + shy;
+          if (shzk != 0.0) dz = dz + shzk;
+          const double dist2 = dx * dx + dy * dy + dz * dz;
+          if (dist2 < rc2) {
+            const double dist_inv = sqrt(1.0 / dist2);
+            double nf;
+            bd::potential<POT>(R.kappa, sigma, dist_inv, pot, nf);
+            const double qx = nf * dist_inv * dx;
+            const double qy = nf * dist_inv * dy;
+            const double qz = nf * dist_inv * dz;
+            fx += qx;
+            fy += qy;
+            fz += qz;
+            vir -= 0.5 * (dx * qx + dy * qy + dz * qz);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int POT, bool SPLIT, class ACC>
+__device__ inline void scan_fast_nr(int nr, const RepDev& R, int n, double cs, double sigma,
+                                    double rc2, int self, double x, double y, double z,
+                                    const int hw[3], const int hu[3], const ACC& acc,
+                                    double& fx, double& fy, double& fz, double& vir,
+                                    double& pot, int sub, int G, int cbase) {
+  if (nr == 1)
+    scan_fast<POT, 1, SPLIT>(R, n, cs, sigma, rc2, self, x, y, z, hw, hu, acc, fx, fy, fz, vir,
+                             pot, sub, G, cbase);
+  else
+    scan_fast<POT, 2, SPLIT>(R, n, cs, sigma, rc2, self, x, y, z, hw, hu, acc, fx, fy, fz, vir,
+                             pot, sub, G, cbase);
+}
+
+// Both stencils of replica-local particle i (same species, then the other
+// species, main.cu:576-585) from global memory, as one rolled pass loop.
+template <int POT, bool SPLIT>
+__device__ inline void forces_global(const RepDev& R, int i, double x, double y, double z,
+                                     const int* __restrict__ cnt,
+                                     const double4* __restrict__ cpos, double& fx, double& fy,
+                                     double& fz, double& v, double& u, int sub = 0, int G = 1) {
+  const int g = i < R.nb ? 0 : 1;
+  int cbase = 0;
+#pragma unroll
+  for (int pass = 0; pass < 2; pass++) {
+    const int gg = pass == 0 ? g : 1 - g;
+    const int nr = pass == 0 ? (g == 0 ? R.nrange_same[0] : R.nrange_same[1])
+                             : (g == 0 ? R.nrange_cross[0] : R.nrange_cross[1]);
+    const double sigma = pass == 0 ? (g == 0 ? R.sigma[0] : R.sigma[1]) : R.sigma_bs;
+    const double rc2 = pass == 0 ? (g == 0 ? R.rc2_same[0] : R.rc2_same[1]) : R.rc2_bs;
+    const int self = pass == 0 ? i : -1;
+    if (R.fast) {
+      int hg[3];
+      if (!cell_coords(R, gg, x, y, z, hg[0], hg[1], hg[2])) return;
+      const int n = gg == 0 ? R.ncell[0] : R.ncell[1];
+      const GlobalCells acc{cnt + (gg == 0 ? R.cell_off[0] : R.cell_off[1]),
+                            cpos + (gg == 0 ? R.tag_off[0] : R.tag_off[1]), n,
+                            gg == 0 ? R.ntags[0] : R.ntags[1]};
+      scan_fast_nr<POT, SPLIT>(nr, R, n, gg == 0 ? R.cell_side[0] : R.cell_side[1], sigma, rc2,
+                               self, x, y, z, hg, hg, acc, fx, fy, fz, v, u, sub, G, cbase);
+      cbase += (2 * nr + 1) * (2 * nr + 1) * (2 * nr + 1);
+    } else {
+      int nc = 0;
+      scan_grid<POT, false>(R, gg, nr, sigma, rc2, self, x, y, z, cnt, cpos, fx, fy, fz, v, u,
+                            nullptr, nc, 0);
+    }
+  }
+}
+
 // Forces, virials and potentials of every particle: same-species stencil
 // then cross-species stencil (main.cu:576-585), accumulators start at zero.
+// NEIGH records the interacting partners in traversal order (test hook).
 template <int POT, bool NEIGH>
 __global__ void __launch_bounds__(kBlk)
     k_forces(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
@@ -366,18 +586,332 @@ __global__ void __launch_bounds__(kBlk)
   const int g = i < R.nb ? 0 : 1;
   const double x = px[p], y = py[p], z = pz[p];
   double fx = 0.0, fy = 0.0, fz = 0.0, v = 0.0, u = 0.0;
-  int nc = 0;
-  int* nl = NEIGH ? nlist + (size_t)i * maxn : nullptr;
-  scan_grid<POT, NEIGH>(R, g, R.nrange_same[g], R.sigma[g], R.rc2_same[g], i, x, y, z, cnt,
-                        cpos, fx, fy, fz, v, u, nl, nc, maxn);
-  scan_grid<POT, NEIGH>(R, 1 - g, R.nrange_cross[g], R.sigma_bs, R.rc2_bs, -1, x, y, z, cnt,
-                        cpos, fx, fy, fz, v, u, nl, nc, maxn);
+  if (NEIGH) {
+    int nc = 0;
+    int* nl = nlist + (size_t)i * maxn;
+    scan_grid<POT, true>(R, g, R.nrange_same[g], R.sigma[g], R.rc2_same[g], i, x, y, z, cnt,
+                         cpos, fx, fy, fz, v, u, nl, nc, maxn);
+    scan_grid<POT, true>(R, 1 - g, R.nrange_cross[g], R.sigma_bs, R.rc2_bs, -1, x, y, z, cnt,
+                         cpos, fx, fy, fz, v, u, nl, nc, maxn);
+    ncounts[i] = nc;
+  } else {
+    forces_global<POT, false>(R, i, x, y, z, cnt, cpos, fx, fy, fz, v, u);
+  }
   ffx[p] = fx;
   ffy[p] = fy;
   ffz[p] = fz;
   vir[p] = v;
   pot[p] = u;
-  if (NEIGH) ncounts[i] = nc;
+}
+
+// FAST force order: G consecutive lanes share one particle, each walking
+// every G-th stencil cell (same-species cells first, then cross-species, in
+// the reference's cell order), and the G partial sums are combined by a
+// fixed xor butterfly.  Deterministic and identical for every replica and
+// launch, but the additions are grouped differently from the reference's
+// single running sum, so results agree with it to rounding (<= ~1e-15
+// relative), not bit for bit.  Needs R.fast for every replica.
+template <int POT, int G>
+__global__ void __launch_bounds__(kBlk)
+    k_forces_split(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+                   const double* __restrict__ px, const double* __restrict__ py,
+                   const double* __restrict__ pz, const int* __restrict__ cnt,
+                   const double4* __restrict__ cpos, double* __restrict__ ffx,
+                   double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
+                   double* __restrict__ pot) {
+  const int4 t = tbl[blockIdx.x];
+  const int i = t.y + (int)threadIdx.x / G;
+  const int sub = (int)threadIdx.x % G;
+  const bool live = i < t.z;
+  const RepDev& R = reps[t.x];
+  double fx = 0.0, fy = 0.0, fz = 0.0, v = 0.0, u = 0.0;
+  if (live) {
+    const int p = R.p_off + i;
+    forces_global<POT, true>(R, i, px[p], py[p], pz[p], cnt, cpos, fx, fy, fz, v, u, sub, G);
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    fx += __shfl_xor(fx, off, G);
+    fy += __shfl_xor(fy, off, G);
+    fz += __shfl_xor(fz, off, G);
+    v += __shfl_xor(v, off, G);
+    u += __shfl_xor(u, off, G);
+  }
+  if (live && sub == 0) {
+    const int p = R.p_off + i;
+    ffx[p] = fx;
+    ffy[p] = fy;
+    ffz[p] = fz;
+    vir[p] = v;
+    pot[p] = u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged brick force kernel.
+//
+// A workgroup owns a brick of B x B x B cells of the big grid.  Its home
+// particles are the big particles binned in those cells and the small
+// particles whose big-grid cell (cell_coords on grid 0) lies in the brick.
+// The workgroup stages into LDS every cell either species can reach from
+// the brick: the big cells of the brick widened by max(nrange_bb, nrange_sb)
+// and the small cells a home particle can occupy widened by
+// max(nrange_ss, nrange_bs).  Cells are addressed by unwrapped coordinates
+// relative to the box and staged row by row (x runs, coalesced).  Home
+// particles are listed in cell order, then each thread walks one particle's
+// two stencils from LDS (scan_fast), so the forces are bit-identical to the
+// global-memory kernels.  A brick that would stage more than CAP positions
+// walks its particles from global memory instead.
+// ---------------------------------------------------------------------------
+constexpr int kBrickThreads = 128;
+constexpr int kBrickMaxCells = 2048;
+constexpr int kBrickMaxRows = 512;
+
+struct BrickBox {
+  int lo[3];
+  int dim[3];
+};
+
+__device__ inline int wrap_cell(int u, int n) {
+  int w = u % n;
+  return w < 0 ? w + n : w;
+}
+
+// exclusive scan of vals[0..n) in place (n <= 4 * kBrickThreads); returns the total
+__device__ inline int block_scan(int* vals, int n, int* part) {
+  const int tid = (int)threadIdx.x;
+  const int chunk = (n + kBrickThreads - 1) / kBrickThreads;
+  const int c0 = min(n, tid * chunk), c1 = min(n, c0 + chunk);
+  int mysum = 0;
+  for (int c = c0; c < c1; c++) mysum += vals[c];
+  part[tid] = mysum;
+  __syncthreads();
+  for (int off = 1; off < kBrickThreads; off <<= 1) {
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  const int total = part[kBrickThreads - 1];
+  int run = part[tid] - mysum;
+  for (int c = c0; c < c1; c++) {
+    const int v = vals[c];
+    vals[c] = run;
+    run += v;
+  }
+  __syncthreads();
+  return total;
+}
+
+template <int POT, int CAP>
+__global__ void __launch_bounds__(kBrickThreads)
+    k_forces_brick(const RepDev* __restrict__ reps, const int4* __restrict__ btbl, int B,
+                   const double* __restrict__ px, const double* __restrict__ py,
+                   const double* __restrict__ pz, const int* __restrict__ cnt,
+                   const double4* __restrict__ cpos,
+                   double* __restrict__ ffx, double* __restrict__ ffy, double* __restrict__ ffz,
+                   double* __restrict__ vir, double* __restrict__ pot, int* __restrict__ flags) {
+  __shared__ unsigned int tbl[kBrickMaxCells];
+  __shared__ double4 spos[CAP];
+  __shared__ int home[CAP];
+  __shared__ int rows[kBrickMaxRows];
+  __shared__ int part[kBrickThreads];
+  const int tid = (int)threadIdx.x;
+  const int4 t = btbl[blockIdx.x];
+  if (t.x < 0) return;  // padding entry of the XCD-grouped table
+  const RepDev& R = reps[t.x];
+  const int b0[3] = {t.y, t.z, t.w};
+  const int rb = max(R.nrange_same[0], R.nrange_cross[1]);
+  const int rs = max(R.nrange_same[1], R.nrange_cross[0]);
+  const int nbg = R.ncell[0], nsg = R.ncell[1];
+  BrickBox bb, sb;
+  int bext[3], shl[3], shh[3];
+  for (int a = 0; a < 3; a++) {
+    bext[a] = min(B, nbg - b0[a]);
+    bb.lo[a] = b0[a] - rb;
+    bb.dim[a] = bext[a] + 2 * rb;
+    // small-grid coordinates a home particle can have (1e-9 covers rounding)
+    const int l = (int)floor((double)b0[a] * R.cell_side[0] * R.cell_inv[1] - 1e-9);
+    const int h = (int)floor((double)(b0[a] + bext[a]) * R.cell_side[0] * R.cell_inv[1] + 1e-9);
+    shl[a] = l;
+    shh[a] = min(h, l + nsg - 1);
+    sb.lo[a] = l - rs;
+    sb.dim[a] = h - l + 2 * rs + 1;
+  }
+  const int nbig = bb.dim[0] * bb.dim[1] * bb.dim[2];
+  const int ncells = nbig + sb.dim[0] * sb.dim[1] * sb.dim[2];
+  const int nbr = bb.dim[1] * bb.dim[2];
+  const int nrows = nbr + sb.dim[1] * sb.dim[2];
+  if (ncells > kBrickMaxCells || nrows > kBrickMaxRows) {  // excluded on the host
+    if (tid == 0) atomicOr(&flags[t.x], kFlagBrick);
+    return;
+  }
+  // 1. occupancy of every staged cell, one x-run (row) per thread
+  for (int r = tid; r < nrows; r += kBrickThreads) {
+    const int g = r < nbr ? 0 : 1;
+    const BrickBox& bx = g == 0 ? bb : sb;
+    const int n = g == 0 ? nbg : nsg;
+    const int ntags = g == 0 ? R.ntags[0] : R.ntags[1];
+    const int rr = g == 0 ? r : r - nbr;
+    const int ry = rr % bx.dim[1], rz = rr / bx.dim[1];
+    const int* crow = cnt + (g == 0 ? R.cell_off[0] : R.cell_off[1]) +
+                      n * (wrap_cell(bx.lo[1] + ry, n) + n * wrap_cell(bx.lo[2] + rz, n));
+    const int cbase = (g == 0 ? 0 : nbig) + bx.dim[0] * rr;
+    int wx = wrap_cell(bx.lo[0], n), sum = 0;
+    for (int rx = 0; rx < bx.dim[0]; rx++) {
+      const int o = crow[wx];
+      const int occ = o < ntags ? o : ntags;
+      tbl[cbase + rx] = (unsigned int)occ;
+      sum += occ;
+      wx = wx + 1 == n ? 0 : wx + 1;
+    }
+    rows[r] = sum;
+  }
+  __syncthreads();
+  // 2. row offsets into the staged position array
+  const int total = block_scan(rows, nrows, part);
+  const bool staged = total <= CAP;
+  // 3. cell offsets and the gather of positions (row by row)
+  for (int r = tid; r < nrows; r += kBrickThreads) {
+    const int g = r < nbr ? 0 : 1;
+    const BrickBox& bx = g == 0 ? bb : sb;
+    const int n = g == 0 ? nbg : nsg;
+    const int ntags = g == 0 ? R.ntags[0] : R.ntags[1];
+    const int rr = g == 0 ? r : r - nbr;
+    const int ry = rr % bx.dim[1], rz = rr / bx.dim[1];
+    const int rowcell = n * (wrap_cell(bx.lo[1] + ry, n) + n * wrap_cell(bx.lo[2] + rz, n));
+    const double4* src = cpos + (g == 0 ? R.tag_off[0] : R.tag_off[1]);
+    const int cbase = (g == 0 ? 0 : nbig) + bx.dim[0] * rr;
+    int wx = wrap_cell(bx.lo[0], n), start = rows[r];
+    for (int rx = 0; rx < bx.dim[0]; rx++) {
+      const int occ = (int)tbl[cbase + rx];
+      tbl[cbase + rx] = ((unsigned int)start << 6) | (unsigned int)occ;
+      if (staged) {
+        const double4* cs = src + (size_t)(rowcell + wx) * ntags;
+        for (int s = 0; s < occ; s++) spos[start + s] = cs[s];
+      }
+      start += occ;
+      wx = wx + 1 == n ? 0 : wx + 1;
+    }
+  }
+  __syncthreads();
+  const LdsCells lb{tbl, spos, bb.lo[0], bb.lo[1], bb.lo[2], bb.dim[0], bb.dim[1]};
+  const LdsCells ls{tbl + nbig, spos, sb.lo[0], sb.lo[1], sb.lo[2], sb.dim[0], sb.dim[1]};
+  // 4. home particles in cell order: big rows of the brick, then small rows
+  //    holding particles whose big cell is in the brick (filtered).  The
+  //    list holds LDS slots (staged) or replica-local indices (not staged).
+  const GlobalCells gb{cnt + R.cell_off[0], cpos + R.tag_off[0], nbg, R.ntags[0]};
+  const GlobalCells gs{cnt + R.cell_off[1], cpos + R.tag_off[1], nsg, R.ntags[1]};
+  const int hb = bext[1] * bext[2];
+  const int sh1 = shh[1] - shl[1] + 1, sh2 = shh[2] - shl[2] + 1;
+  const int nhr = hb + sh1 * sh2;
+#pragma unroll 1
+  for (int pass = 0; pass < 2; pass++) {  // 0: count, 1: write
+#pragma unroll 1
+    for (int r = tid; r < nhr; r += kBrickThreads) {
+      int at = pass == 1 ? rows[r] : 0;
+      const bool big = r < hb;
+      int ry, rz, x0, x1;
+      if (big) {
+        ry = b0[1] + r % bext[1];
+        rz = b0[2] + r / bext[1];
+        x0 = b0[0];
+        x1 = b0[0] + bext[0] - 1;
+      } else {
+        ry = shl[1] + (r - hb) % sh1;
+        rz = shl[2] + (r - hb) / sh1;
+        x0 = shl[0];
+        x1 = shh[0];
+      }
+#pragma unroll 1
+      for (int cx = x0; cx <= x1; cx++) {
+        int occ, st = 0;
+        const double4* cell;
+        if (staged) {
+          const LdsCells& lc = big ? lb : ls;
+          const unsigned int e = lc.tbl[lc.index(cx, ry, rz)];
+          occ = (int)(e & 63u);
+          st = (int)(e >> 6);
+          cell = spos + st;
+        } else {
+          const GlobalCells& gc = big ? gb : gs;
+          gc.cell(gc.index(cx, ry, rz), occ, cell);
+        }
+#pragma unroll 1
+        for (int s = 0; s < occ; s++) {
+          const double4 q = cell[s];
+          if (!big) {
+            int ii, jj, kk;
+            if (!cell_coords(R, 0, q.x, q.y, q.z, ii, jj, kk)) continue;
+            if (ii < b0[0] || ii >= b0[0] + bext[0] || jj < b0[1] || jj >= b0[1] + bext[1] ||
+                kk < b0[2] || kk >= b0[2] + bext[2])
+              continue;
+          }
+          if (pass == 1 && at < CAP) home[at] = staged ? st + s : (int)q.w;
+          at++;
+        }
+      }
+      if (pass == 0) rows[r] = at;
+    }
+    __syncthreads();
+    if (pass == 0) {
+      const int nh = block_scan(rows, nhr, part);
+      if (nh > CAP) {  // cannot happen below 32 particles per cell
+        if (tid == 0) atomicOr(&flags[t.x], kFlagBrick);
+        return;
+      }
+      if (tid == 0) part[0] = nh;  // keep the count (part is free after the scan)
+      __syncthreads();
+    }
+  }
+  const int nhome = part[0];
+  // 5. forces of the home particles in the reference order
+#pragma unroll 1
+  for (int h = tid; h < nhome; h += kBrickThreads) {
+    double fx = 0.0, fy = 0.0, fz = 0.0, v = 0.0, u = 0.0;
+    int i;
+    if (staged) {
+      const double4 me = spos[home[h]];
+      i = (int)me.w;
+      const int g = i < R.nb ? 0 : 1;
+      int hwb[3], hws[3], hus[3];
+      cell_coords(R, 0, me.x, me.y, me.z, hwb[0], hwb[1], hwb[2]);
+      cell_coords(R, 1, me.x, me.y, me.z, hws[0], hws[1], hws[2]);
+      for (int a = 0; a < 3; a++) {
+        int uu = hws[a];
+        if (uu < shl[a]) uu += nsg;
+        if (uu >= shl[a] + nsg) uu -= nsg;
+        hus[a] = uu;
+      }
+#pragma unroll 1
+      for (int pass = 0; pass < 2; pass++) {
+        const int gg = pass == 0 ? g : 1 - g;
+        const int nr = pass == 0 ? (g == 0 ? R.nrange_same[0] : R.nrange_same[1])
+                                 : (g == 0 ? R.nrange_cross[0] : R.nrange_cross[1]);
+        const double sigma = pass == 0 ? (g == 0 ? R.sigma[0] : R.sigma[1]) : R.sigma_bs;
+        const double rc2 = pass == 0 ? (g == 0 ? R.rc2_same[0] : R.rc2_same[1]) : R.rc2_bs;
+        const LdsCells acc = gg == 0 ? lb : ls;
+        const int hw[3] = {gg == 0 ? hwb[0] : hws[0], gg == 0 ? hwb[1] : hws[1],
+                           gg == 0 ? hwb[2] : hws[2]};
+        const int hu[3] = {gg == 0 ? hwb[0] : hus[0], gg == 0 ? hwb[1] : hus[1],
+                           gg == 0 ? hwb[2] : hus[2]};
+        scan_fast_nr<POT, false>(nr, R, gg == 0 ? nbg : nsg,
+                                 gg == 0 ? R.cell_side[0] : R.cell_side[1], sigma, rc2,
+                                 pass == 0 ? i : -1, me.x, me.y, me.z, hw, hu, acc, fx, fy, fz, v,
+                                 u, 0, 1, 0);
+      }
+    } else {
+      i = home[h];
+      const int p = R.p_off + i;
+      forces_global<POT, false>(R, i, px[p], py[p], pz[p], cnt, cpos, fx, fy, fz, v, u);
+    }
+    const int p = R.p_off + i;
+    ffx[p] = fx;
+    ffy[p] = fy;
+    ffz[p] = fz;
+    vir[p] = v;
+    pot[p] = u;
+  }
 }
 
 // Per-replica reductions for fself_isotropic (fself.cu:3-21), the MSD
@@ -421,6 +955,51 @@ __global__ void __launch_bounds__(kObsThreads)
     for (int k = 0; k < 6; k++) out[blockIdx.x * 6 + k] = red[k][0];
 }
 
+// F_s(q) and MSD of every replica against every stored waiting-time origin
+// (raw positions saved by bd_set_origin): workgroup (replica, slot),
+// fixed-order tree reduction.
+constexpr int kOriginThreads = 256;
+__global__ void __launch_bounds__(kOriginThreads)
+    k_fself_origins(const RepDev* __restrict__ reps, const double* __restrict__ rx,
+                    const double* __restrict__ ry, const double* __restrict__ rz,
+                    const double* __restrict__ org, size_t slot_stride, int n_total, double q,
+                    double* __restrict__ out) {
+  const int rep = (int)blockIdx.x, slot = (int)blockIdx.y;
+  const RepDev& R = reps[rep];
+  const double* ox = org + slot * slot_stride;
+  const double* oy = ox + n_total;
+  const double* oz = oy + n_total;
+  double fs = 0.0, mb = 0.0, ms = 0.0;
+  for (int i = (int)threadIdx.x; i < R.n; i += kOriginThreads) {
+    const int p = R.p_off + i;
+    const double dx = rx[p] - ox[p], dy = ry[p] - oy[p], dz = rz[p] - oz[p];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (i < R.nb) {
+      const double a = q * sqrt(d2);
+      fs += a == 0.0 ? 1.0 : sin(a) / a;
+      mb += d2;
+    } else {
+      ms += d2;
+    }
+  }
+  __shared__ double red[3][kOriginThreads];
+  red[0][threadIdx.x] = fs;
+  red[1][threadIdx.x] = mb;
+  red[2][threadIdx.x] = ms;
+  __syncthreads();
+  for (int w = kOriginThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const size_t o = ((size_t)rep * gridDim.y + slot) * 3;
+    out[o] = red[0][0] / (double)R.nb;
+    out[o + 1] = red[1][0] / (double)R.nb;
+    out[o + 2] = red[2][0] / (double)(R.n - R.nb);
+  }
+}
+
 // Ordered-pair distance histograms of one replica (get_gder.cu:69-127):
 // pair (i, j), i != j, histogram 2*species(i) + species(j).  j tiles are
 // staged in LDS, counts go to wavefront-private LDS histograms (integer
@@ -432,7 +1011,13 @@ __global__ void __launch_bounds__(kGderThreads)
            const double* __restrict__ py, const double* __restrict__ pz, int nbins,
            double range, double bin_size, int j_chunk,
            unsigned long long* __restrict__ counts) {
+  // rep < 0: one replica per blockIdx.z, counts offset by 4*nbins per replica
+  if (rep < 0) {
+    rep = (int)blockIdx.z;
+    counts += (size_t)rep * 4 * nbins;
+  }
   const RepDev& R = reps[rep];
+  if ((int)blockIdx.x * kGderThreads >= R.n || (int)blockIdx.y * j_chunk >= R.n) return;
   extern __shared__ unsigned int hist[];  // [waves][4][nbins]
   __shared__ double tx[kGderThreads], ty[kGderThreads], tz[kGderThreads];
   const int waves = kGderThreads / 64;
@@ -556,6 +1141,16 @@ struct bd_sys {
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
+  // brick force kernel (0 = thread-per-particle global kernel)
+  int brick = 0;
+  std::vector<int4> tbl_brick_h;
+  int4* tbl_brick = nullptr;
+  // force summation order: 1 = reference (bit-exact), G = 4 or 8 = split
+  int order = 1;
+  std::vector<int4> tbl_split_h;
+  int4* tbl_split = nullptr;
+  // waiting-time origins: BD_MAX_ORIGINS slots of raw positions (x, y, z)
+  double* origins = nullptr;
 };
 
 namespace {
@@ -603,8 +1198,103 @@ void launch_forces_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* 
   }
 }
 
+template <int CAP>
+void launch_brick_t(bd_sys* s) {
+  const int nblk = (int)s->tbl_brick_h.size();
+  switch (s->pot_kind) {
+    case BD_POT_WCA:
+      k_forces_brick<BD_POT_WCA, CAP><<<nblk, kBrickThreads, 0, s->stream>>>(
+          s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy,
+          s->fz, s->vir, s->pot, s->flags);
+      break;
+    case BD_POT_WCA_MODIFIED:
+      k_forces_brick<BD_POT_WCA_MODIFIED, CAP><<<nblk, kBrickThreads, 0, s->stream>>>(
+          s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy,
+          s->fz, s->vir, s->pot, s->flags);
+      break;
+    default:
+      k_forces_brick<BD_POT_HARMONIC, CAP><<<nblk, kBrickThreads, 0, s->stream>>>(
+          s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy,
+          s->fz, s->vir, s->pot, s->flags);
+      break;
+  }
+}
+
+template <int G>
+void launch_split_t(bd_sys* s) {
+  const int nblk = (int)s->tbl_split_h.size();
+  switch (s->pot_kind) {
+    case BD_POT_WCA:
+      k_forces_split<BD_POT_WCA, G><<<nblk, kBlk, 0, s->stream>>>(
+          s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy, s->fz,
+          s->vir, s->pot);
+      break;
+    case BD_POT_WCA_MODIFIED:
+      k_forces_split<BD_POT_WCA_MODIFIED, G><<<nblk, kBlk, 0, s->stream>>>(
+          s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy, s->fz,
+          s->vir, s->pot);
+      break;
+    default:
+      k_forces_split<BD_POT_HARMONIC, G><<<nblk, kBlk, 0, s->stream>>>(
+          s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy, s->fz,
+          s->vir, s->pot);
+      break;
+  }
+}
+
 void launch_forces(bd_sys* s) {
-  launch_forces_t<false>(s, nullptr, nullptr, 0, s->tbl_part, (int)s->tbl_part_h.size());
+  if (s->order == 4) {
+    launch_split_t<4>(s);
+  } else if (s->order == 8) {
+    launch_split_t<8>(s);
+  } else if (s->brick == 0) {
+    launch_forces_t<false>(s, nullptr, nullptr, 0, s->tbl_part, (int)s->tbl_part_h.size());
+  } else if (s->brick <= 3) {
+    launch_brick_t<1024>(s);
+  } else {
+    launch_brick_t<2048>(s);
+  }
+}
+
+// Largest number of cells a brick of B big cells stages (both boxes).
+int brick_cells_bound(const RepDev& R, int B) {
+  const int rb = std::max(R.nrange_same[0], R.nrange_cross[1]);
+  const int rs = std::max(R.nrange_same[1], R.nrange_cross[0]);
+  const int bd = B + 2 * rb;
+  const int sd = (int)((double)B * R.cell_side[0] * R.cell_inv[1]) + 3 + 2 * rs;
+  return bd * bd * bd + sd * sd * sd;
+}
+
+// Brick table: every replica tiled by B^3 big-cell bricks, then grouped so
+// that consecutive bricks (spatial neighbours sharing halo cells) land on
+// the same XCD (workgroups are dealt round-robin over the 8 XCDs).
+int setup_bricks(bd_sys* s, int B) {
+  if (B == 0) {
+    s->brick = 0;
+    return BD_OK;
+  }
+  if (B < 1 || B > 4) return fail(BD_E_ARG, "brick must be 0..4");
+  std::vector<int4> natural;
+  for (int r = 0; r < s->R; r++) {
+    const RepDev& R = s->reps[r];
+    if (!R.fast || B > R.ncell[0] || brick_cells_bound(R, B) > kBrickMaxCells)
+      return fail(BD_E_ARG, "brick too large for replica " + std::to_string(r));
+    for (int bz = 0; bz < R.ncell[0]; bz += B)
+      for (int by = 0; by < R.ncell[0]; by += B)
+        for (int bx = 0; bx < R.ncell[0]; bx += B) natural.push_back(make_int4(r, bx, by, bz));
+  }
+  const int M = (int)natural.size();
+  const int C = (M + 7) / 8;
+  std::vector<int4> tbl((size_t)C * 8, make_int4(-1, 0, 0, 0));
+  for (int k = 0; k < M; k++) tbl[(size_t)(k % C) * 8 + k / C] = natural[k];
+  if (s->tbl_brick) (void)hipFree(s->tbl_brick);
+  s->tbl_brick = nullptr;
+  int rc = dev_alloc(&s->tbl_brick, tbl.size());
+  if (rc != BD_OK) return rc;
+  HIP_TRY(hipMemcpy(s->tbl_brick, tbl.data(), sizeof(int4) * tbl.size(), hipMemcpyHostToDevice));
+  s->tbl_brick_h = tbl;
+  s->brick = B;
+  return BD_OK;
 }
 
 void enqueue_step(bd_sys* s) {
@@ -621,6 +1311,8 @@ int check_flags(bd_sys* s) {
   for (int r = 0; r < s->R; r++) {
     if (f[r] & kFlagNonFinite)
       return fail(BD_E_NAN, "replica " + std::to_string(r) + ": non-finite position");
+    if (f[r] & kFlagBrick)
+      return fail(BD_E_STATE, "replica " + std::to_string(r) + ": brick exceeds its LDS table");
     if (f[r] & kFlagOverflow)
       return fail(BD_E_CELL_OVERFLOW,
                   "replica " + std::to_string(r) + ": a cell reached ntags particles");
@@ -636,6 +1328,9 @@ int check_rep(const bd_sys* s, int r) {
 
 void free_all(bd_sys* s) {
   if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  if (s->tbl_brick) (void)hipFree(s->tbl_brick);
+  if (s->tbl_split) (void)hipFree(s->tbl_split);
+  if (s->origins) (void)hipFree(s->origins);
   void* ptrs[] = {s->tbl_part, s->tbl_lane, s->tbl_cell, s->reps_d, s->px,   s->py,
                   s->pz,       s->rx,       s->ry,       s->rz,     s->ix,   s->iy,
                   s->iz,       s->fx,       s->fy,       s->fz,     s->vir,  s->pot,
@@ -646,12 +1341,33 @@ void free_all(bd_sys* s) {
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
-void add_blocks(std::vector<int4>& tbl, int rep, int count, int grid) {
-  for (int first = 0; first < count; first += kBlk)
-    tbl.push_back(make_int4(rep, first, std::min(count, first + kBlk), grid));
+void add_blocks(std::vector<int4>& tbl, int rep, int count, int grid, int per_block = kBlk) {
+  for (int first = 0; first < count; first += per_block)
+    tbl.push_back(make_int4(rep, first, std::min(count, first + per_block), grid));
 }
 
 }  // namespace
+
+namespace bdi {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+int device_state(bd_sys* s, int replica, DevState* out) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (replica < 0 || replica >= s->R) return fail(BD_E_ARG, "replica index out of range");
+  const RepDev& R = s->reps[replica];
+  out->px = s->px + R.p_off;
+  out->py = s->py + R.p_off;
+  out->pz = s->pz + R.p_off;
+  out->fx = s->fx + R.p_off;
+  out->fy = s->fy + R.p_off;
+  out->fz = s->fz + R.p_off;
+  out->n = R.n;
+  out->nb = R.nb;
+  out->side = R.side;
+  out->stream = s->stream;
+  return BD_OK;
+}
+}  // namespace bdi
 
 // ===========================================================================
 // C ABI
@@ -786,10 +1502,16 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
     R.nrange_cross[1] = g.nrange_sb;
     R.potential = p.potential;
     R.temp_protocol = p.temp_protocol;
+    {
+      const int rb = std::max(g.nrange_bb, g.nrange_sb), rs = std::max(g.nrange_ss, g.nrange_bs);
+      R.fast = rb <= 2 && rs <= 2 && g.ncell_big >= 2 * rb + 3 && g.ncell_sml >= 2 * rs + 3;
+    }
     R.side = p.side;
     R.side_inv = 1.0 / p.side;
     R.cell_inv[0] = 1.0 / g.cell_side_big;
     R.cell_inv[1] = 1.0 / g.cell_side_sml;
+    R.cell_side[0] = g.cell_side_big;
+    R.cell_side[1] = g.cell_side_sml;
     R.sigma[0] = p.sigma_big;
     R.sigma[1] = p.sigma_sml;
     R.sigma_bs = 0.5 * (p.sigma_big + p.sigma_sml);
@@ -878,6 +1600,9 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
     delete s;
     return fail(BD_E_HIP, "bd_create: " + m);
   }
+  // default force kernel: thread per particle from global memory (the
+  // autotuner may pick an LDS brick where it measures faster)
+  s->brick = 0;
   *out = s;
   return BD_OK;
 }
@@ -1154,6 +1879,137 @@ int bd_step(bd_sys* s, int32_t nsteps) {
   return check_flags(s);
 }
 
+int bd_set_brick(bd_sys* s, int32_t brick) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->graph) {
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_steps = 0;
+  }
+  return setup_bricks(s, brick);
+}
+
+int bd_set_force_order(bd_sys* s, int32_t order) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (order != BD_ORDER_REFERENCE && order != BD_ORDER_SPLIT4 && order != BD_ORDER_SPLIT8)
+    return fail(BD_E_ARG, "order must be BD_ORDER_REFERENCE, _SPLIT4 or _SPLIT8");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->graph) {
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_steps = 0;
+  }
+  if (order != BD_ORDER_REFERENCE) {
+    for (int r = 0; r < s->R; r++)
+      if (!s->reps[r].fast)
+        return fail(BD_E_ARG, "split order needs grids of at least 2*nrange+3 cells");
+    std::vector<int4> tbl;
+    for (int r = 0; r < s->R; r++) add_blocks(tbl, r, s->reps[r].n, 0, kBlk / order);
+    if (s->tbl_split) (void)hipFree(s->tbl_split);
+    s->tbl_split = nullptr;
+    int rc = dev_alloc(&s->tbl_split, tbl.size());
+    if (rc != BD_OK) return rc;
+    HIP_TRY(hipMemcpy(s->tbl_split, tbl.data(), sizeof(int4) * tbl.size(), hipMemcpyHostToDevice));
+    s->tbl_split_h = tbl;
+  }
+  s->order = order;
+  return BD_OK;
+}
+
+// Autotuner: average duration of the force kernel for every brick shape the
+// engine accepts (all give bit-identical results), forces-only launches on
+// the current cells so the simulation state is unchanged.  Choices are cached
+// per (replicas, N bucket, phi bucket, potential, order) in a text file.
+static std::string tune_key(const bd_sys* s) {
+  double vol_p = 0.0, vol_box = 0.0;
+  for (const bd_params& p : s->params) {
+    vol_p += M_PI / 6.0 *
+             (p.n_big * std::pow(p.sigma_big, 3.0) + p.n_sml * std::pow(p.sigma_sml, 3.0));
+    vol_box += p.side * p.side * p.side;
+  }
+  const int nb = (int)std::lround(4.0 * std::log2((double)std::max(1, s->n_total)));
+  char buf[160];
+  std::snprintf(buf, sizeof buf, "R%d_N%d_phi%.2f_pot%d_order%d", s->R, nb, vol_p / vol_box,
+                s->pot_kind, s->order);
+  return buf;
+}
+
+int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32_t* chosen) {
+  if (!s || trial_launches < 1) return fail(BD_E_ARG, "bad arguments");
+  const std::string key = tune_key(s);
+  if (cache_path) {
+    if (FILE* f = std::fopen(cache_path, "r")) {
+      char k[200];
+      int b;
+      int found = -1;
+      while (std::fscanf(f, "%199s %d", k, &b) == 2)
+        if (key == k) found = b;
+      std::fclose(f);
+      if (found >= 0 && bd_set_brick(s, found) == BD_OK) {
+        if (chosen) *chosen = found;
+        return BD_OK;
+      }
+    }
+  }
+  int rc = require_positions(s);
+  if (rc != BD_OK) return rc;
+  if (!s->located) {
+    launch_bin(s);
+    launch_cells(s, 0);
+    s->located = true;
+  }
+  int best = 0;
+  double best_ms = 1e300;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const int candidates[] = {0, 1, 2, 3, 4};
+  const int keep = s->brick;
+  for (int b : candidates) {
+    if (s->order != BD_ORDER_REFERENCE && b != 0) continue;
+    if (setup_bricks(s, b) != BD_OK) continue;
+    launch_forces(s);  // warm-up
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    for (int k = 0; k < trial_launches; k++) launch_forces(s);
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = b;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  g_err.clear();
+  rc = bd_set_brick(s, best);
+  if (rc != BD_OK) {
+    bd_set_brick(s, keep);
+    return rc;
+  }
+  if (cache_path)
+    if (FILE* f = std::fopen(cache_path, "a")) {
+      std::fprintf(f, "%s %d\n", key.c_str(), best);
+      std::fclose(f);
+    }
+  if (chosen) *chosen = best;
+  return BD_OK;
+}
+
+int bd_get_force_order(const bd_sys* s, int32_t* order) {
+  if (!s || !order) return fail(BD_E_ARG, "null argument");
+  *order = s->order;
+  return BD_OK;
+}
+
+int bd_get_brick(const bd_sys* s, int32_t* brick) {
+  if (!s || !brick) return fail(BD_E_ARG, "null argument");
+  *brick = s->brick;
+  return BD_OK;
+}
+
 int bd_kernel_times(bd_sys* s, int32_t nsteps, double* ms) {
   if (!s || !ms || nsteps < 0) return fail(BD_E_ARG, "bad arguments");
   int rc = require_positions(s);
@@ -1253,6 +2109,70 @@ int bd_gder(bd_sys* s, int32_t replica, int32_t nbins, double range, uint64_t* c
   (void)hipFree(c_d);
   if (e != hipSuccess) return fail(BD_E_HIP, std::string("bd_gder: ") + hipGetErrorString(e));
   for (size_t k = 0; k < h.size(); k++) counts[k] += h[k];
+  return BD_OK;
+}
+
+int bd_set_origin(bd_sys* s, int32_t slot) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (slot < 0 || slot >= BD_MAX_ORIGINS) return fail(BD_E_ARG, "origin slot out of range");
+  const size_t n = (size_t)s->n_total;
+  if (!s->origins) {
+    int rc = dev_alloc(&s->origins, (size_t)BD_MAX_ORIGINS * 3 * n);
+    if (rc != BD_OK) return rc;
+  }
+  double* o = s->origins + (size_t)slot * 3 * n;
+  HIP_TRY(hipMemcpyAsync(o, s->rx, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(o + n, s->ry, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(o + 2 * n, s->rz, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return BD_OK;
+}
+
+int bd_fself_origins(bd_sys* s, double q, int32_t nslots, double* out) {
+  if (!s || !out) return fail(BD_E_ARG, "null argument");
+  if (nslots < 1 || nslots > BD_MAX_ORIGINS) return fail(BD_E_ARG, "nslots out of range");
+  if (!s->origins) return fail(BD_E_STATE, "no origin stored");
+  double* o_d = nullptr;
+  const size_t cnt = (size_t)s->R * nslots * 3;
+  int rc = dev_alloc(&o_d, cnt);
+  if (rc != BD_OK) return rc;
+  k_fself_origins<<<dim3(s->R, nslots), kOriginThreads, 0, s->stream>>>(
+      s->reps_d, s->rx, s->ry, s->rz, s->origins, (size_t)3 * s->n_total, s->n_total, q, o_d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, o_d, cnt * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(o_d);
+  if (e != hipSuccess) return fail(BD_E_HIP, std::string("bd_fself_origins: ") + hipGetErrorString(e));
+  return BD_OK;
+}
+
+int bd_gder_all(bd_sys* s, int32_t nbins, double range, uint64_t* counts) {
+  if (!s || !counts) return fail(BD_E_ARG, "null argument");
+  if (nbins < 1 || nbins > kGderMaxBins || !(range > 0.0))
+    return fail(BD_E_ARG, "nbins must be in 1..512 and range > 0");
+  int nmax = 0;
+  for (const RepDev& R : s->reps) nmax = std::max(nmax, R.n);
+  const size_t cnt = (size_t)s->R * 4 * nbins;
+  unsigned long long* c_d = nullptr;
+  int rc = dev_alloc(&c_d, cnt);
+  if (rc != BD_OK) return rc;
+  hipError_t e = hipMemsetAsync(c_d, 0, sizeof(unsigned long long) * cnt, s->stream);
+  const int iblocks = (nmax + kGderThreads - 1) / kGderThreads;
+  const int j_chunk = 2048;
+  const int jsplit = (nmax + j_chunk - 1) / j_chunk;
+  const size_t shmem = sizeof(unsigned int) * (kGderThreads / 64) * 4 * nbins;
+  if (e == hipSuccess) {
+    k_gder<<<dim3(iblocks, jsplit, s->R), kGderThreads, shmem, s->stream>>>(
+        s->reps_d, -1, s->px, s->py, s->pz, nbins, range, range / (double)nbins, j_chunk, c_d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(counts, c_d, sizeof(unsigned long long) * cnt, hipMemcpyDeviceToHost,
+                       s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(c_d);
+  if (e != hipSuccess) return fail(BD_E_HIP, std::string("bd_gder_all: ") + hipGetErrorString(e));
   return BD_OK;
 }
 

@@ -173,6 +173,29 @@ class Engine:
     def step(self, nsteps=1):
         check(self.lib.bd_step(self.h, int(nsteps)))
 
+    def autotune(self, trial_launches=10, cache_path=None):
+        """Pick the fastest force-kernel shape (results are bit-identical)."""
+        v = C.c_int32()
+        path = cache_path.encode() if cache_path else None
+        check(self.lib.bd_autotune(self.h, int(trial_launches), path, C.byref(v)))
+        return v.value
+
+    def set_force_order(self, order):
+        check(self.lib.bd_set_force_order(self.h, int(order)))
+
+    def force_order(self):
+        v = C.c_int32()
+        check(self.lib.bd_get_force_order(self.h, C.byref(v)))
+        return v.value
+
+    def set_brick(self, brick):
+        check(self.lib.bd_set_brick(self.h, int(brick)))
+
+    def brick(self):
+        v = C.c_int32()
+        check(self.lib.bd_get_brick(self.h, C.byref(v)))
+        return v.value
+
     def kernel_times(self, nsteps):
         """ms spent in (update, cells, forces) over nsteps eager steps."""
         ms = np.zeros(3)
@@ -193,6 +216,20 @@ class Engine:
         check(self.lib.bd_gder(self.h, r, int(nbins), float(rng), _uptr(c)))
         return c.reshape(4, nbins)
 
+    def gder_counts_all(self, nbins, rng):
+        c = np.zeros(self.R * 4 * nbins, dtype=np.uint64)
+        check(self.lib.bd_gder_all(self.h, int(nbins), float(rng), _uptr(c)))
+        return c.reshape(self.R, 4, nbins)
+
+    def set_origin(self, slot):
+        check(self.lib.bd_set_origin(self.h, int(slot)))
+
+    def fself_origins(self, q, nslots):
+        """(R, nslots, 3): F_s(q) big, MSD big, MSD small vs each origin."""
+        out = np.zeros(self.R * nslots * 3)
+        check(self.lib.bd_fself_origins(self.h, float(q), int(nslots), _dptr(out)))
+        return out.reshape(self.R, nslots, 3)
+
     def observables(self, q, r=0):
         o = BDObs()
         check(self.lib.bd_observables(self.h, r, float(q), C.byref(o)))
@@ -208,3 +245,47 @@ def rng_stream(seed, lanes, ndraws):
 
 def lane_seed(seed, lane):
     return load().bd_lane_seed(int(seed) & ((1 << 64) - 1), int(lane))
+
+
+class Contraction:
+    """Force contraction (Contracting_Vector3D): add samples, solve, read."""
+
+    def __init__(self, bins, rng):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.bd_cv3d_create(int(bins), float(rng), C.byref(h)))
+        self.h = h
+        self.bins = int(bins)
+
+    def close(self):
+        if self.h:
+            self.lib.bd_cv3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, engine, r=0, periodic=True):
+        check(self.lib.bd_cv3d_add(self.h, engine.h, int(r), 1 if periodic else 0))
+
+    def add_host(self, xyz, f, side=0.0):
+        a = np.ascontiguousarray(xyz, dtype=np.float64)
+        b = np.ascontiguousarray(f, dtype=np.float64)
+        check(self.lib.bd_cv3d_add_host(self.h, _dptr(a), _dptr(b), len(a), float(side)))
+
+    def solve(self):
+        check(self.lib.bd_cv3d_solve(self.h))
+
+    def vector(self):
+        out = np.zeros(self.bins)
+        check(self.lib.bd_cv3d_get(self.h, _dptr(out)))
+        return out
+
+    def normal(self):
+        ata = np.zeros((3, self.bins, self.bins))
+        atb = np.zeros((3, self.bins))
+        check(self.lib.bd_cv3d_get_normal(self.h, _dptr(ata), _dptr(atb)))
+        return ata, atb

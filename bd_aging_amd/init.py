@@ -5,12 +5,38 @@ eliminate_overlap binaries that are not part of it).
 Uniform random placement (one rannew64 stream, host) followed by overdamped
 overlap removal on the GPU engine: BD steps at T = 0 with the harmonic
 potential U = kappa/2 (sigma_ij - r)^2, r < sigma_ij, until the largest
-overlap is below `tol * sigma_sml`.
+overlap is below `tol * sigma_sml`.  The result is put in Z-order per
+species (spatial_sort).
 """
 import numpy as np
 
 from .abi import BD_POT_HARMONIC
 from .engine import Engine, make_params, random_positions
+
+
+def _spread_bits(v):
+    v = v.astype(np.uint64) & np.uint64(0x3ff)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x030000ff)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x0300f00f)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x030c30c3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x09249249)
+    return v
+
+
+def morton_order(x, side):
+    """Permutation putting positions in Z-order of a 1024^3 grid (stable)."""
+    q = np.clip((x / side * 1024.0).astype(np.int64), 0, 1023)
+    code = _spread_bits(q[:, 0]) | (_spread_bits(q[:, 1]) << np.uint64(1)) | \
+        (_spread_bits(q[:, 2]) << np.uint64(2))
+    return np.argsort(code, kind="stable")
+
+
+def spatial_sort(x, n_big, side):
+    """Reorder each species along the same Z-order curve, so that particles
+    with neighbouring indices are neighbours in space (wavefront lanes then
+    share cache lines in the cell gathers).  Species stay big first."""
+    xb, xs = x[:n_big], x[n_big:]
+    return np.concatenate([xb[morton_order(xb, side)], xs[morton_order(xs, side)]])
 
 
 def relax_overlaps(n_big, n_sml, side, sigma_big=1.4, sigma_sml=1.0, seed=1, kappa=1.0,
@@ -35,7 +61,7 @@ def relax_overlaps(n_big, n_sml, side, sigma_big=1.4, sigma_sml=1.0, seed=1, kap
                 break
         else:
             raise RuntimeError(f"overlaps remain after {max_steps} steps (max U {u.max():.3e})")
-        return eng.positions()
+        return spatial_sort(eng.positions(), n_big, side)
 
 
 def relax_batch(specs, kappa=1.0, dt=0.1, tol=1e-3, max_steps=200000, chunk=256):
@@ -63,4 +89,5 @@ def relax_batch(specs, kappa=1.0, dt=0.1, tol=1e-3, max_steps=200000, chunk=256)
                 break
         else:
             raise RuntimeError(f"overlaps remain after {max_steps} steps")
-        return [eng.positions(r) for r in range(len(specs))]
+        return [spatial_sort(eng.positions(r), specs[r][0], specs[r][2])
+                for r in range(len(specs))]

@@ -136,6 +136,32 @@ int bd_update(bd_sys* s);
  * (main.cu:494-586); replayed through a captured HIP graph. */
 int bd_step(bd_sys* s, int32_t nsteps);
 
+/* Force-kernel shape (no reference counterpart; the autotuner's knob):
+ * brick = 1..4 selects the LDS-staged kernel over brick^3 big-grid cells,
+ * 0 the thread-per-particle kernel.  Every choice gives bit-identical
+ * results. */
+int bd_set_brick(bd_sys* s, int32_t brick);
+int bd_get_brick(const bd_sys* s, int32_t* brick);
+
+/* Autotuner (no reference counterpart): measures the force kernel for every
+ * brick shape (bit-identical results by construction) with forces-only
+ * launches on the current state, keeps the fastest and returns it in
+ * *chosen.  With cache_path != NULL the choice is looked up in / appended to
+ * that text file, keyed by (replicas, N, phi, potential, order). */
+int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32_t* chosen);
+
+/* Force summation order (no reference counterpart).
+ * BD_ORDER_REFERENCE (default): each particle's pair terms are added in the
+ *   reference's traversal order (get_forces.cu:360-416, 495-550), so forces
+ *   are bit-identical to the FP64 restatement of the reference.
+ * BD_ORDER_SPLIT4 / _SPLIT8: 4 or 8 lanes share a particle's stencil and
+ *   their partial sums are combined in a fixed order: deterministic, equal to
+ *   the reference order to rounding (~1e-15 relative), faster on small
+ *   systems.  Needs grids of at least 2*nrange+3 cells. */
+enum { BD_ORDER_REFERENCE = 1, BD_ORDER_SPLIT4 = 4, BD_ORDER_SPLIT8 = 8 };
+int bd_set_force_order(bd_sys* s, int32_t order);
+int bd_get_force_order(const bd_sys* s, int32_t* order);
+
 /* Measurement hook (no reference counterpart): runs nsteps steps with the
  * kernels launched one by one on the engine's stream and HIP events around
  * each; adds the elapsed milliseconds of the update, cells and forces kernels
@@ -165,6 +191,65 @@ int bd_random_positions(int32_t n, double side, uint64_t seed, double* xyz);
  * integer counts added into counts[4 * nbins]. */
 int bd_gder(bd_sys* s, int32_t replica, int32_t nbins, double range, uint64_t* counts);
 int bd_observables(bd_sys* s, int32_t replica, double q, bd_obs* out);
+/* The same histograms for every replica in one launch:
+ * counts[r * 4 * nbins + ...] (overwritten, not added). */
+int bd_gder_all(bd_sys* s, int32_t nbins, double range, uint64_t* counts);
+
+/* Waiting-time origins (no reference counterpart: the reference re-runs the
+ * binary per decade, run-fself.sh:149-197).  bd_set_origin stores the
+ * current raw positions of every replica in `slot`; bd_fself_origins
+ * computes, in one launch, F_s(q) of the big species and the MSD of both
+ * species of every replica against origins 0..nslots-1:
+ * out[(r * nslots + k) * 3 + {0: F_s, 1: msd_big, 2: msd_sml}]. */
+enum { BD_MAX_ORIGINS = 8 };
+int bd_set_origin(bd_sys* s, int32_t slot);
+int bd_fself_origins(bd_sys* s, double q, int32_t nslots, double* out);
+
+/* ---- force contraction (Contracting_Vector3D.cuh:39-75) ---- */
+typedef struct bd_cv3d bd_cv3d;
+/* Contracting_Vector3D(bins, range) (Contracting_Vector3D.cu:11-55) */
+int bd_cv3d_create(int32_t bins, double range, bd_cv3d** out);
+int bd_cv3d_destroy(bd_cv3d* cv);
+/* add_statistics(rr, ff, n, side) (Contracting_Vector3D.cu:57-164) on the
+ * positions and forces of one replica of an engine; periodic != 0 applies
+ * the replica's box as `side`. */
+int bd_cv3d_add(bd_cv3d* cv, bd_sys* s, int32_t replica, int32_t periodic);
+/* the same on host arrays (xyz, f: n x 3); side <= 0 means no images */
+int bd_cv3d_add_host(bd_cv3d* cv, const double* xyz, const double* f, int32_t n, double side);
+/* calculate_vector (Contracting_Vector3D.cu:166-297): BD_E_STATE with no samples */
+int bd_cv3d_solve(bd_cv3d* cv);
+/* get_vector (Contracting_Vector3D.cu:299-304): BD_E_STATE before a solve */
+int bd_cv3d_get(const bd_cv3d* cv, double* out);
+/* test hook: the accumulated normal equations, ata[3*bins*bins], atb[3*bins] */
+int bd_cv3d_get_normal(bd_cv3d* cv, double* ata, double* atb);
+int bd_cv3d_samples(const bd_cv3d* cv, int32_t* out);
+
+/* ---- learned surrogate (no reference counterpart) ----
+ * An ensemble of `members` MLPs n_in -> hidden -> hidden -> n_out with GELU,
+ * BF16 weights/activations on the matrix cores with FP32 accumulation, FP32
+ * master weights and Adam.  Parameters per member, FP32, row-major (x @ W):
+ * W1[n_in][hidden] b1[hidden] W2[hidden][hidden] b2[hidden]
+ * W3[hidden][n_out] b3[n_out]. */
+typedef struct bd_mlp bd_mlp;
+int bd_mlp_create(int32_t members, int32_t n_in, int32_t hidden, int32_t n_out, uint64_t seed,
+                  bd_mlp** out);
+int bd_mlp_destroy(bd_mlp* s);
+int bd_mlp_num_params(const bd_mlp* s, int64_t* per_member);
+int bd_mlp_get_params(bd_mlp* s, float* params);
+int bd_mlp_set_params(bd_mlp* s, const float* params);
+/* y[m][b][o] of every member for rows x[b][i] */
+int bd_mlp_forward(bd_mlp* s, const float* x, int32_t batch, float* y);
+/* mean-squared-error gradients of every member on one batch (test hook) */
+int bd_mlp_gradients(bd_mlp* s, const float* x, const float* t, int32_t batch, float* grads,
+                     float* loss);
+int bd_mlp_train_step(bd_mlp* s, const float* x, const float* t, int32_t batch, float lr,
+                      float* loss);
+/* device-resident training set and the full Adam loop over it */
+int bd_mlp_set_data(bd_mlp* s, const float* x, const float* t, int32_t n);
+int bd_mlp_train(bd_mlp* s, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                 float* losses);
+/* ensemble mean and standard deviation over the members */
+int bd_mlp_predict(bd_mlp* s, const float* x, int32_t n, float* mean, float* stdev);
 
 /* ---- errors ---- */
 const char* bd_last_error(void);

@@ -1,0 +1,97 @@
+"""Force contraction on the GPU against the oracle (Contracting_Vector3D)."""
+import numpy as np
+import pytest
+
+import orc
+from bd_aging_amd import abi
+from bd_aging_amd.engine import Contraction, Engine, make_params, box_side
+from bd_aging_amd.init import relax_overlaps
+
+pytestmark = pytest.mark.gpu
+
+
+def test_normal_equations_bit_exact_host_inputs():
+    rng = np.random.default_rng(0)
+    bins, rr = 40, 3.0
+    c = Contraction(bins, rr)
+    ata_o = np.zeros((3, bins, bins))
+    atb_o = np.zeros((3, bins))
+    for k, (n, side) in enumerate([(500, 9.0), (300, 0.0)]):
+        x = rng.random((n, 3)) * 9.0
+        f = rng.normal(size=(n, 3))
+        c.add_host(x, f, side)
+        a, b = orc.cv3d_accumulate(bins, rr, x, f, side)
+        ata_o += a if k == 0 else 0
+        atb_o += b if k == 0 else 0
+        if k == 1:
+            # the oracle continues from the first sample's sums
+            import ctypes as C
+            A = np.ascontiguousarray(ata_prev)
+            B = np.ascontiguousarray(atb_prev)
+            assert orc.lib().orc_cv3d_accumulate(bins, rr, orc.dp(np.ascontiguousarray(x)),
+                                                 orc.dp(np.ascontiguousarray(f)), n, side,
+                                                 orc.dp(A), orc.dp(B)) == 0
+            ata_o, atb_o = A, B
+        ata_prev, atb_prev = ata_o.copy(), atb_o.copy()
+        ata_g, atb_g = c.normal()
+        assert np.array_equal(ata_g, ata_o), k
+        assert np.array_equal(atb_g, atb_o), k
+    c.solve()
+    got = c.vector()
+    exp = orc.cv3d_solve(ata_o, atb_o)
+    assert np.allclose(got, exp, rtol=1e-9, atol=1e-9 * np.abs(exp).max())
+    c.close()
+
+
+def test_contraction_of_engine_state_matches_oracle():
+    side = box_side(400, 400, 1.4, 1.0, 0.45)
+    x = relax_overlaps(400, 400, side, seed=8)
+    p = make_params(400, 400, side=side, seed=3)
+    e = Engine(p)
+    e.set_positions(x)
+    e.step(10)
+    pos = e.positions()
+    f, _, _ = e.forces()
+    c = Contraction(64, 3.2)
+    c.add(e, 0, periodic=True)
+    ata, atb = orc.cv3d_accumulate(64, 3.2, pos, f, side)
+    ga, gb = c.normal()
+    assert np.array_equal(ga, ata) and np.array_equal(gb, atb)
+    c.solve()
+    exp = orc.cv3d_solve(ata, atb)
+    got = c.vector()
+    assert np.allclose(got, exp, rtol=1e-8, atol=1e-8 * np.abs(exp).max())
+    c.close()
+    e.close()
+
+
+def test_recovers_binwise_radial_force():
+    rng = np.random.default_rng(9)
+    n, side, bins, rr = 300, 9.0, 8, 3.0
+    x = rng.random((n, 3)) * side
+    f_true = np.linspace(2.0, -1.0, bins)
+    d = x[:, None, :] - x[None, :, :]
+    d -= side * np.floor(d / side + 0.5)
+    dist = np.sqrt((d ** 2).sum(-1))
+    np.fill_diagonal(dist, np.inf)
+    F = np.zeros((n, 3))
+    for m in range(n):
+        ok = dist[m] < rr
+        b = np.minimum((dist[m, ok] * (bins / rr)).astype(int), bins - 1)
+        F[m] = (f_true[b][:, None] * d[m, ok] / dist[m, ok][:, None]).sum(0)
+    c = Contraction(bins, rr)
+    c.add_host(x, F, side)
+    c.solve()
+    assert np.allclose(c.vector(), f_true, rtol=1e-8, atol=1e-8)
+    c.close()
+
+
+def test_out_of_order_calls_are_errors():
+    c = Contraction(8, 2.0)
+    with pytest.raises(abi.BDError) as e1:
+        c.solve()
+    assert e1.value.code == abi.BD_E_STATE
+    with pytest.raises(abi.BDError) as e2:
+        c.vector()
+    assert e2.value.code == abi.BD_E_STATE
+    c.close()

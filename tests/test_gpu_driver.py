@@ -1,0 +1,94 @@
+"""The drop-in driver (bd_aging_self) against a replay of the main.cu flow on
+the FP64 oracle: same init_config files, same stdin protocol, same samples."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import orc
+from bd_aging_amd import abi
+from bd_aging_amd.engine import make_params, lane_seed
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "bd_aging_amd", "bd_aging_self")
+
+PROTOCOL = """linear
+1.0  0.1  0.0  0.0  1
+1e-4
+linear
+0.002  0.006  3
+8 8
+2
+40 4.0
+7.0
+4321
+"""
+
+
+def read_config(path):
+    lines = open(path).read().splitlines()
+    nb, ns, nt = (int(v) for v in lines[2].split()[:3])
+    side = float(lines[1].split()[0])
+    sb, ss = (float(v) for v in lines[3].split()[:2])
+    xyz = np.array([[float(v) for v in ln.split()[1:4]] for ln in lines[6:6 + nt]])
+    return nb, ns, side, sb, ss, xyz
+
+
+def test_driver_matches_oracle_replay(tmp_path):
+    cfg = tmp_path / "configs"
+    sim = tmp_path / "sim"
+    (sim / "results").mkdir(parents=True)
+    r = subprocess.run([EXE, "--make-configs", "250", "250", "0.45", "1.4", "1.0", "2", "77",
+                        str(cfg), "1.0", "50", "1e-4"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = subprocess.run([EXE], input=PROTOCOL, cwd=sim, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "TERMINADO" in r.stdout
+    # replay on the oracle
+    dt, ntrans, niter, nsamples, nbins, rng, q = 1e-4, 20, 60, 3, 40, 4.0, 7.0
+    samples = list(range(0, niter, niter // nsamples))
+    acc = {k: np.zeros(len(samples)) for k in ("fs", "mb", "ms", "eb", "es", "z", "T")}
+    gsum = np.zeros((len(samples), 4, nbins))
+    for i in range(2):
+        nb, ns, side, sb, ss, xyz = read_config(cfg / f"init_config_{i + 1}")
+        p = make_params(nb, ns, side=side, sigma_big=sb, sigma_sml=ss, dt=dt,
+                        potential=abi.BD_POT_WCA_MODIFIED, T0=1.0, Tf=0.1, t0=0.0, tf=0.0,
+                        seed=lane_seed(4321, i), first_step=-ntrans)
+        o = orc.System(p)
+        o.set_positions(xyz)
+        assert o.cell_locate() == 0
+        o.compute_forces()
+        assert o.step(ntrans) == 0
+        o.mark_origin()
+        done = 0
+        for k, target in enumerate(samples):
+            assert o.step(target + 1 - done) == 0
+            done = target + 1
+            ob = o.observables(q)
+            acc["fs"][k] += ob["fself_big"]
+            acc["mb"][k] += ob["msd_big"]
+            acc["ms"][k] += ob["msd_sml"]
+            acc["eb"][k] += ob["epot_big"]
+            acc["es"][k] += ob["epot_sml"]
+            acc["z"][k] += ob["z"]
+            acc["T"][k] = ob["temperature"]
+            gsum[k] += orc.gder_normalize(p, nbins, rng, o.gder_counts(nbins, rng))
+    res = sim / "results"
+    t = dt * (1.0 + np.array(samples, float))
+    fs = np.loadtxt(res / "fself.out")
+    assert np.allclose(fs[:, 0], t, atol=2e-6)
+    assert np.allclose(fs[:, 1], acc["fs"] / 2, atol=2e-6)
+    msd = np.loadtxt(res / "msd_averaged.out")
+    assert np.allclose(msd[:, 1:], np.stack([acc["mb"], acc["ms"]], 1) / 2, atol=2e-6)
+    en = np.loadtxt(res / "energies_averaged.out")
+    assert np.allclose(en[:, 1:3], np.stack([acc["eb"], acc["es"]], 1) / 2, atol=2e-6)
+    assert np.allclose(en[:, 3], acc["T"], atol=2e-6)
+    pr = np.loadtxt(res / "pressure.out")
+    assert np.allclose(pr[:, 1], acc["z"] / 2, rtol=1e-6, atol=2e-6)
+    for k in range(len(samples)):
+        g = np.loadtxt(res / f"gder_{k}.out")
+        assert g.shape == (nbins, 5)
+        assert np.allclose(g[:, 1:], (gsum[k] / 2).T, atol=2e-6, rtol=1e-6)

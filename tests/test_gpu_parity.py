@@ -115,6 +115,34 @@ def test_forces_and_neighbours(config1_positions, pot):
     e.close()
 
 
+@pytest.mark.parametrize("pot", [BD_POT_WCA, BD_POT_WCA_MODIFIED])
+def test_every_brick_shape_bit_identical(config1_positions, pot):
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, potential=pot, seed=4)
+    o = orc.System(p)
+    o.set_positions(x)
+    o.cell_locate()
+    o.compute_forces()
+    fo, vo, uo = o.forces()
+    e = Engine(p)
+    e.set_positions(x)
+    e.cell_locate()
+    tried = []
+    for b in (0, 1, 2, 3, 4):
+        try:
+            e.set_brick(b)
+        except abi.BDError as err:  # shape whose staged box exceeds the LDS table
+            assert err.code == abi.BD_E_ARG
+            continue
+        tried.append(b)
+        assert e.brick() == b
+        e.compute_forces()
+        fe, ve, ue = e.forces()
+        assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and np.array_equal(ue, uo), b
+    assert {0, 1, 2, 3} <= set(tried)
+    e.close()
+
+
 # ------------------------------------------------------------------ update
 def test_one_update_bit_exact(config1_positions):
     side, x = config1_positions
@@ -279,3 +307,59 @@ def test_relaxed_configuration_has_no_overlaps(config1_positions):
     big = np.arange(2000) < 1000
     sig = np.where(big[:, None] & big[None, :], 1.4, np.where(~big[:, None] & ~big[None, :], 1.0, 1.2))
     assert (r / sig).min() >= 1.0 - 1.5e-3
+
+
+# ------------------------------------------------------------------ split order
+@pytest.mark.parametrize("order", [abi.BD_ORDER_SPLIT4, abi.BD_ORDER_SPLIT8])
+def test_split_order_forces_and_trajectory(config1_positions, order):
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, T0=1.0, Tf=0.3, t0=0.0, tf=0.0, seed=21)
+    o = orc.System(p)
+    o.set_positions(x)
+    o.cell_locate()
+    o.compute_forces()
+    fo, vo, uo = o.forces()
+    e = Engine(p)
+    e.set_positions(x)
+    e.set_force_order(order)
+    assert e.force_order() == order
+    e.cell_locate()
+    e.compute_forces()
+    fe, ve, ue = e.forces()
+    assert_forces_close(fe, fo)
+    assert_forces_close(ve, vo)
+    assert_forces_close(ue, uo)
+    e.step(100)
+    assert o.step(100) == 0
+    d = np.abs(e.positions() - o.positions())
+    assert np.minimum(d, side - d).max() <= TRAJ_TOL
+    # deterministic: a second engine reproduces the run bit for bit
+    e2 = Engine(p)
+    e2.set_positions(x)
+    e2.set_force_order(order)
+    e2.step(100)
+    assert np.array_equal(e.positions(), e2.positions())
+    e.close()
+    e2.close()
+
+
+def test_autotune_keeps_results_and_caches(config1_positions, tmp_path):
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, seed=2)
+    e = Engine(p)
+    e.set_positions(x)
+    e.cell_locate()
+    e.compute_forces()
+    f0 = e.forces()
+    cache = str(tmp_path / "tune.txt")
+    b = e.autotune(5, cache)
+    assert e.brick() == b
+    e.compute_forces()
+    f1 = e.forces()
+    assert all(np.array_equal(a, c) for a, c in zip(f0, f1))
+    assert open(cache).read().split()[-1] == str(b)
+    e2 = Engine(p)
+    e2.set_positions(x)
+    assert e2.autotune(5, cache) == b  # served from the cache
+    e.close()
+    e2.close()
