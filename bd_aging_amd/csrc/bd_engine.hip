@@ -161,6 +161,7 @@ __global__ void k_rng_seed(const RepDev* __restrict__ reps, const int4* __restri
                            uint64_t* __restrict__ ring, uint64_t* __restrict__ lcg,
                            int total_lanes) {
   const int4 t = tbl[blockIdx.x];
+  if (t.x < 0) return;  // padding entry of an XCD-grouped table
   const int l = t.y + (int)threadIdx.x;
   if (l >= t.z) return;
   const RepDev& R = reps[t.x];
@@ -181,6 +182,7 @@ __global__ void __launch_bounds__(kBlk)
           const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ ins,
           int* __restrict__ flags) {
   const int4 t = tbl[blockIdx.x];
+  if (t.x < 0) return;
   const int i = t.y + (int)threadIdx.x;
   if (i >= t.z) return;
   const RepDev& R = reps[t.x];
@@ -202,6 +204,7 @@ __global__ void __launch_bounds__(kBlk)
                  const int64_t* __restrict__ step, int total_lanes, int* __restrict__ nocup,
                  int* __restrict__ ins, int* __restrict__ flags, int do_bin) {
   const int4 t = tbl[blockIdx.x];
+  if (t.x < 0) return;
   const int l = t.y + (int)threadIdx.x;
   if (l >= t.z) return;
   const RepDev& R = reps[t.x];
@@ -269,6 +272,7 @@ __global__ void __launch_bounds__(kBlk)
     for (int r = 0; r < n_reps; r++) draws[r] += 4ULL * (uint64_t)reps[r].per_lane;
   }
   const int4 t = tbl[blockIdx.x];
+  if (t.x < 0) return;
   const int c = t.y + (int)threadIdx.x;
   if (c >= t.z) return;
   const int g = t.w;
@@ -579,6 +583,7 @@ __global__ void __launch_bounds__(kBlk)
              double* __restrict__ pot, int* __restrict__ ncounts, int* __restrict__ nlist,
              int maxn) {
   const int4 t = tbl[blockIdx.x];
+  if (t.x < 0) return;
   const int i = t.y + (int)threadIdx.x;
   if (i >= t.z) return;
   const RepDev& R = reps[t.x];
@@ -620,6 +625,7 @@ __global__ void __launch_bounds__(kBlk)
                    double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
                    double* __restrict__ pot) {
   const int4 t = tbl[blockIdx.x];
+  if (t.x < 0) return;  // whole-block padding entry (uniform)
   const int i = t.y + (int)threadIdx.x / G;
   const int sub = (int)threadIdx.x % G;
   const bool live = i < t.z;
@@ -1140,7 +1146,8 @@ struct bd_sys {
   bool located = false;
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
-  int graph_steps = 0;
+  int graph_steps = 0;       // steps in the instantiated graph
+  int graph_mode = 16;       // steps per graph launch; 0 = eager
   // brick force kernel (0 = thread-per-particle global kernel)
   int brick = 0;
   std::vector<int4> tbl_brick_h;
@@ -1265,6 +1272,18 @@ int brick_cells_bound(const RepDev& R, int B) {
   return bd * bd * bd + sd * sd * sd;
 }
 
+// Deal consecutive table entries (same replica, neighbouring particles or
+// cells) to the same XCD: workgroups are dispatched round-robin over the 8
+// XCDs, so entry k goes to slot (k % C) * 8 + k / C; padding entries have
+// replica -1.
+std::vector<int4> xcd_group(const std::vector<int4>& natural) {
+  const int M = (int)natural.size();
+  const int C = (M + 7) / 8;
+  std::vector<int4> tbl((size_t)C * 8, make_int4(-1, 0, 0, 0));
+  for (int k = 0; k < M; k++) tbl[(size_t)(k % C) * 8 + k / C] = natural[k];
+  return tbl;
+}
+
 // Brick table: every replica tiled by B^3 big-cell bricks, then grouped so
 // that consecutive bricks (spatial neighbours sharing halo cells) land on
 // the same XCD (workgroups are dealt round-robin over the 8 XCDs).
@@ -1283,10 +1302,7 @@ int setup_bricks(bd_sys* s, int B) {
       for (int by = 0; by < R.ncell[0]; by += B)
         for (int bx = 0; bx < R.ncell[0]; bx += B) natural.push_back(make_int4(r, bx, by, bz));
   }
-  const int M = (int)natural.size();
-  const int C = (M + 7) / 8;
-  std::vector<int4> tbl((size_t)C * 8, make_int4(-1, 0, 0, 0));
-  for (int k = 0; k < M; k++) tbl[(size_t)(k % C) * 8 + k / C] = natural[k];
+  const std::vector<int4> tbl = xcd_group(natural);
   if (s->tbl_brick) (void)hipFree(s->tbl_brick);
   s->tbl_brick = nullptr;
   int rc = dev_alloc(&s->tbl_brick, tbl.size());
@@ -1534,6 +1550,9 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
     for (int k = 0; k < 2; k++)
       add_blocks(s->tbl_cell_h, r, R.ncell[k] * R.ncell[k] * R.ncell[k], k);
   }
+  s->tbl_part_h = xcd_group(s->tbl_part_h);
+  s->tbl_lane_h = xcd_group(s->tbl_lane_h);
+  s->tbl_cell_h = xcd_group(s->tbl_cell_h);
   if (p_off > INT32_MAX / 2 || tag_off > INT32_MAX || lane_off > INT32_MAX / 2) {
     delete s;
     return fail(BD_E_ARG, "engine too large for 32-bit offsets");
@@ -1861,9 +1880,9 @@ int bd_step(bd_sys* s, int32_t nsteps) {
     launch_forces(s);
     s->located = true;
   }
-  const int G = 16;
+  const int G = s->graph_mode;
   int left = nsteps;
-  if (left >= G) {
+  if (G > 0 && left >= G) {
     if (!s->graph || s->graph_steps != G) {
       rc = build_graph(s, G);
       if (rc != BD_OK) return rc;
@@ -1877,6 +1896,25 @@ int bd_step(bd_sys* s, int32_t nsteps) {
   HIP_TRY(hipGetLastError());
   s->ni += nsteps;
   return check_flags(s);
+}
+
+int bd_set_graph_steps(bd_sys* s, int32_t steps) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (steps < 0 || steps > 256) return fail(BD_E_ARG, "graph steps must be 0..256");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->graph) {
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_steps = 0;
+  }
+  s->graph_mode = steps;
+  return BD_OK;
+}
+
+int bd_get_graph_steps(const bd_sys* s, int32_t* steps) {
+  if (!s || !steps) return fail(BD_E_ARG, "null argument");
+  *steps = s->graph_mode;
+  return BD_OK;
 }
 
 int bd_set_brick(bd_sys* s, int32_t brick) {
@@ -1906,6 +1944,7 @@ int bd_set_force_order(bd_sys* s, int32_t order) {
         return fail(BD_E_ARG, "split order needs grids of at least 2*nrange+3 cells");
     std::vector<int4> tbl;
     for (int r = 0; r < s->R; r++) add_blocks(tbl, r, s->reps[r].n, 0, kBlk / order);
+    tbl = xcd_group(tbl);
     if (s->tbl_split) (void)hipFree(s->tbl_split);
     s->tbl_split = nullptr;
     int rc = dev_alloc(&s->tbl_split, tbl.size());

@@ -188,6 +188,15 @@ class Engine:
         check(self.lib.bd_get_force_order(self.h, C.byref(v)))
         return v.value
 
+    def set_graph_steps(self, steps):
+        """Steps per captured graph in step(); 0 enqueues every kernel eagerly."""
+        check(self.lib.bd_set_graph_steps(self.h, int(steps)))
+
+    def graph_steps(self):
+        v = C.c_int32()
+        check(self.lib.bd_get_graph_steps(self.h, C.byref(v)))
+        return v.value
+
     def set_brick(self, brick):
         check(self.lib.bd_set_brick(self.h, int(brick)))
 

@@ -212,3 +212,79 @@ class Surrogate:
         """Ensemble mean, std and the fallback mask (True: run plain BD)."""
         mean, std = self.net.predict((X - self.mu) / self.sd)
         return mean, std, std.max(1) > self.std_gate
+
+
+def evaluate(sur, X, Y, bd_seconds_per_sample):
+    """RMS of the ensemble mean against BD-measured F_s, with and without the
+    std gate, and the cost ratio of simulating versus predicting."""
+    t0 = time.perf_counter()
+    mean, std, fallback = sur.predict(X)
+    pred_s = (time.perf_counter() - t0) / len(X)
+    err = mean - Y
+    rms_all = float(np.sqrt(np.mean(err ** 2)))
+    keep = ~fallback
+    rms_kept = float(np.sqrt(np.mean(err[keep] ** 2))) if keep.any() else float("nan")
+    # the gated answer: surrogate where trusted, BD itself where it falls back
+    gated = np.where(fallback[:, None], Y, mean)
+    rms_gated = float(np.sqrt(np.mean((gated - Y) ** 2)))
+    return {"rms_all": rms_all, "rms_accepted": rms_kept, "rms_with_fallback": rms_gated,
+            "fallback_fraction": float(fallback.mean()), "samples": int(len(X)),
+            "predict_seconds_per_sample": pred_s,
+            "bd_seconds_per_sample": bd_seconds_per_sample,
+            "cost_ratio_bd_over_surrogate": bd_seconds_per_sample / pred_s,
+            "cost_ratio_with_fallback": bd_seconds_per_sample /
+            (pred_s + float(fallback.mean()) * bd_seconds_per_sample)}
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import os
+    import sys
+    ap = argparse.ArgumentParser(description="train and evaluate the F_s surrogate")
+    ap.add_argument("--runs", type=int, default=20)
+    ap.add_argument("--test-runs", type=int, default=2)
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--nmin", type=int, default=1000)
+    ap.add_argument("--nmax", type=int, default=16000)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--out", default="surrogate_out")
+    a = ap.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+
+    def log(m):
+        print(m, flush=True)
+
+    xs, ys, walls, costs = [], [], [], []
+    for k in range(a.runs + a.test_runs):
+        X, Y, cost, wall = generate(R=a.replicas, n_range=(a.nmin, a.nmax), seed=1000 + k,
+                                    log=log)
+        xs.append(X)
+        ys.append(Y)
+        walls.append(wall)
+        costs.append(cost)
+    Xtr, Ytr = np.concatenate(xs[:a.runs]), np.concatenate(ys[:a.runs])
+    Xte, Yte = np.concatenate(xs[a.runs:]), np.concatenate(ys[a.runs:])
+    log(f"[surrogate] train {len(Xtr)} samples, test {len(Xte)}")
+    sur = Surrogate()
+    t0 = time.perf_counter()
+    losses = sur.fit(Xtr, Ytr, epochs=a.epochs)
+    train_s = time.perf_counter() - t0
+    log(f"[surrogate] trained in {train_s:.1f} s, final losses {losses[-1]}")
+    bd_per_sample = sum(walls[a.runs:]) / len(Xte)
+    rep = evaluate(sur, Xte, Yte, bd_per_sample)
+    rep.update({"train_samples": int(len(Xtr)), "epochs": a.epochs, "train_seconds": train_s,
+                "final_train_loss": losses[-1].tolist(), "lags": LAGS.tolist(), "k": K_FS,
+                "members": 5, "std_gate": sur.std_gate,
+                "bd_particle_steps_per_s": sum(costs) / sum(walls)})
+    log(json.dumps(rep))
+    json.dump(rep, open(os.path.join(a.out, "report.json"), "w"), indent=1)
+    np.savez(os.path.join(a.out, "surrogate_model.npz"), params=sur.net.params(), mu=sur.mu,
+             sd=sur.sd, lags=LAGS)
+    np.savez_compressed(os.path.join(a.out, "test_curves.npz"), Y=Yte.astype(np.float32),
+                        P=sur.predict(Xte)[0])
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

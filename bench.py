@@ -44,7 +44,7 @@ def batch_specs(R=256, seed=20251203):
     return n, phi, T
 
 
-def run_plain(steps, warmup):
+def run_plain(steps, warmup, graph_steps=16):
     nb = ns = 16384
     side = box_side(nb, ns, 1.4, 1.0, 0.58)
     log(f"[bench] relaxing N=32768 phi=0.58 L={side:.4f}")
@@ -52,6 +52,7 @@ def run_plain(steps, warmup):
     p = make_params(nb, ns, side=side, dt=1e-4, T0=1.0, Tf=0.1, t0=0.0, tf=0.0,
                     first_step=-warmup, seed=777)
     eng = Engine(p)
+    eng.set_graph_steps(graph_steps)
     eng.set_positions(x)
     log(f"[bench] warmup {warmup} steps at T=1.0")
     eng.step(warmup)
@@ -68,7 +69,7 @@ def run_plain(steps, warmup):
             "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p}
 
 
-def run_batch(steps, warmup):
+def run_batch(steps, warmup, graph_steps=16):
     n, phi, T = batch_specs()
     specs, params = [], []
     for r in range(len(n)):
@@ -81,6 +82,7 @@ def run_batch(steps, warmup):
     log(f"[bench] relaxing 256 replicas, sum N = {int(n.sum())}")
     xs = relax_batch(specs)
     eng = Engine(params)
+    eng.set_graph_steps(graph_steps)
     for r, x in enumerate(xs):
         eng.set_positions(x, r)
     eng.step(warmup)
@@ -125,16 +127,22 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph-steps", type=int, default=16,
+                    help="steps per captured HIP graph; 0 launches every kernel eagerly "
+                         "(same kernels, for kernel tracing)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the runs, copy a 256 MiB buffer 3 times (a plain copy kernel of "
+                         "known traffic, the scale for FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: replicas shard across GPUs with no per-step "
               "exchange, so N>1 is not run here", file=sys.stderr)
         sys.exit(2)
-    plain = run_plain(a.steps, a.warmup)
+    plain = run_plain(a.steps, a.warmup, a.graph_steps)
     log(f"[bench] plain: {plain['rate']:.4e} particle-steps/s, kernel ms {plain['kernel_ms']}")
     batch = None
     if not a.no_batch:
-        batch = run_batch(a.batch_steps, min(a.warmup, 500))
+        batch = run_batch(a.batch_steps, min(a.warmup, 500), a.graph_steps)
         log(f"[bench] batch: {batch['rate']:.4e} particle-steps/s")
     cpu = None if a.no_cpu else cpu_baseline(plain)
     N = plain["N"]
@@ -172,6 +180,19 @@ def main():
                         "step_hbm_frac": B_STEP * batch["rate"] / HBM_PEAK,
                         "kernel_ms": dict(zip(["update", "cells", "forces"], batch["kernel_ms"]))}
     print(json.dumps(out), flush=True)
+    if a.copy_calibration:
+        copy_calibration()
+
+
+def copy_calibration(nbytes=256 << 20, reps=3):
+    import torch
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    src.uniform_()
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: {reps} x {nbytes} B read + {nbytes} B written")
 
 
 if __name__ == "__main__":

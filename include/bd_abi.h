@@ -136,6 +136,13 @@ int bd_update(bd_sys* s);
  * (main.cu:494-586); replayed through a captured HIP graph. */
 int bd_step(bd_sys* s, int32_t nsteps);
 
+/* Launch mode of bd_step (no reference counterpart): steps per captured
+ * HIP graph, 1..256 (default 16), or 0 to enqueue every kernel eagerly (for
+ * tracing tools that do not follow graph launches).  Results are
+ * bit-identical in every mode. */
+int bd_set_graph_steps(bd_sys* s, int32_t steps);
+int bd_get_graph_steps(const bd_sys* s, int32_t* steps);
+
 /* Force-kernel shape (no reference counterpart; the autotuner's knob):
  * brick = 1..4 selects the LDS-staged kernel over brick^3 big-grid cells,
  * 0 the thread-per-particle kernel.  Every choice gives bit-identical

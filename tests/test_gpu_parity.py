@@ -13,7 +13,7 @@ import pytest
 
 import orc
 from bd_aging_amd import abi
-from bd_aging_amd.abi import BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC, BD_TEMP_SINE
+from bd_aging_amd.abi import BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC, BD_TEMP_SINE, BDError
 from bd_aging_amd.engine import Engine, make_params, rng_stream, box_side
 from bd_aging_amd.init import relax_overlaps
 
@@ -224,6 +224,32 @@ def test_graph_replay_equals_eager(config1_positions):
     assert np.array_equal(a.positions(), b.positions())
     a.close()
     b.close()
+
+
+def test_graph_modes_bit_identical(config1_positions):
+    """Eager launches, 7-step and 16-step graphs give the same trajectory."""
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, seed=12)
+    out = []
+    for g in (0, 7, 16):
+        e = Engine(p)
+        assert e.graph_steps() == 16
+        e.set_graph_steps(g)
+        assert e.graph_steps() == g
+        e.set_positions(x)
+        e.step(40)
+        out.append((e.positions(), e.forces()[0], e.step_index()))
+        e.close()
+    for pos, f, ni in out[1:]:
+        assert np.array_equal(pos, out[0][0])
+        assert np.array_equal(f, out[0][1])
+        assert ni == out[0][2]
+    e = Engine(p)
+    with pytest.raises(BDError):
+        e.set_graph_steps(-1)
+    with pytest.raises(BDError):
+        e.set_graph_steps(257)
+    e.close()
 
 
 # ------------------------------------------------------------------ batch
