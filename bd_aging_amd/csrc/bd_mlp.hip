@@ -39,6 +39,7 @@ struct bd_mlp {
   float *dx = nullptr, *dt = nullptr;
   int n_data = 0;
   int* idx = nullptr;
+  float* lhist = nullptr;  // per-batch losses of one training epoch
   hipStream_t stream = nullptr;
 };
 
@@ -281,10 +282,11 @@ void forward(bd_mlp* s, int batch) {
 
 // Backward of the mean-squared error against s->t (per member targets);
 // writes the gradients of every member into s->g.
-void backward(bd_mlp* s, int batch) {
+// The members' losses go to loss_out[0..members).
+void backward(bd_mlp* s, int batch, float* loss_out) {
   const int M = s->members, H = s->hidden, I = s->n_in, O = s->n_out;
   const long P = (long)s->nparam;
-  k_loss_grad<<<M, kEw, 0, s->stream>>>(s->y, s->t, 1, batch, O, s->dy, s->dyb, s->loss);
+  k_loss_grad<<<M, kEw, 0, s->stream>>>(s->y, s->t, 1, batch, O, s->dy, s->dyb, loss_out);
   // W3: h2^T dy ; b3: colsum(dy)
   gemm(s, s->h2b, (long)batch * H, 1, H, s->dyb, (long)batch * O, O, 1, s->g + s->off_w3, P, O, 1,
        H, O, batch);
@@ -394,7 +396,7 @@ int bd_mlp_destroy(bd_mlp* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   void* ptrs[] = {s->p,  s->g,   s->m1,  s->m2,  s->wb,   s->loss, s->x,    s->t,  s->z1,
                   s->z2, s->y,   s->dy,  s->dh,  s->dz,   s->xb,   s->h1b,  s->h2b, s->dyb,
-                  s->dz2b, s->dz1b, s->dx, s->dt, s->idx};
+                  s->dz2b, s->dz1b, s->dx, s->dt, s->idx, s->lhist};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -457,7 +459,7 @@ int bd_mlp_gradients(bd_mlp* s, const float* x, const float* t, int32_t batch, f
                            s->stream));
   }
   forward(s, batch);
-  backward(s, batch);
+  backward(s, batch, s->loss);
   MLP_TRY(hipGetLastError());
   if (grads)
     MLP_TRY(hipMemcpyAsync(grads, s->g, (size_t)s->members * s->nparam * sizeof(float),
@@ -493,9 +495,11 @@ int bd_mlp_train(bd_mlp* s, int32_t epochs, int32_t batch, float lr, uint64_t se
   if (rc != BD_OK) return rc;
   const int M = s->members, nb = s->n_data / batch;
   if (s->idx) (void)hipFree(s->idx), s->idx = nullptr;
+  if (s->lhist) (void)hipFree(s->lhist), s->lhist = nullptr;
   MLP_TRY(hipMalloc((void**)&s->idx, (size_t)M * s->n_data * sizeof(int)));
+  MLP_TRY(hipMalloc((void**)&s->lhist, (size_t)M * nb * sizeof(float)));
   std::vector<int> perm((size_t)M * s->n_data);
-  std::vector<float> lsum((size_t)M), lb((size_t)M);
+  std::vector<float> lsum((size_t)M), lb((size_t)M * nb);
   Mix r{seed};
   for (int ep = 0; ep < epochs; ep++) {
     for (int m = 0; m < M; m++) {
@@ -520,13 +524,16 @@ int bd_mlp_train(bd_mlp* s, int32_t epochs, int32_t batch, float lr, uint64_t se
             s->dt, im, batch, s->n_out, s->t + (size_t)m * batch * s->n_out);
       }
       forward(s, batch);
-      backward(s, batch);
+      backward(s, batch, s->lhist + (size_t)b * M);
       adam(s, lr);
-      MLP_TRY(hipMemcpyAsync(lb.data(), s->loss, M * sizeof(float), hipMemcpyDeviceToHost,
-                             s->stream));
-      MLP_TRY(hipStreamSynchronize(s->stream));
-      for (int m = 0; m < M; m++) lsum[m] += lb[m];
     }
+    // one read-back per epoch; it also keeps the host's shuffle of the next
+    // epoch from overwriting perm before this epoch's copy has landed
+    MLP_TRY(hipMemcpyAsync(lb.data(), s->lhist, lb.size() * sizeof(float),
+                           hipMemcpyDeviceToHost, s->stream));
+    MLP_TRY(hipStreamSynchronize(s->stream));
+    for (int b = 0; b < nb; b++)
+      for (int m = 0; m < M; m++) lsum[m] += lb[(size_t)b * M + m];
     if (losses)
       for (int m = 0; m < M; m++) losses[(size_t)ep * M + m] = lsum[m] / (float)nb;
   }

@@ -40,8 +40,9 @@ def spatial_sort(x, n_big, side):
 
 
 def relax_overlaps(n_big, n_sml, side, sigma_big=1.4, sigma_sml=1.0, seed=1, kappa=1.0,
-                   dt=0.1, tol=1e-3, max_steps=200000, chunk=256, start=None):
-    """Return an N x 3 array with no pair closer than sigma_ij (1 - tol)."""
+                   dt=0.1, tol=1e-3, max_steps=200000, chunk=256, start=None, graph_steps=16):
+    """Return an N x 3 array with no pair closer than sigma_ij (1 - tol).
+    graph_steps: the engine's launch mode (steps per HIP graph, 0 = eager)."""
     p = make_params(n_big, n_sml, side=side, sigma_big=sigma_big, sigma_sml=sigma_sml,
                     potential=BD_POT_HARMONIC, kappa=kappa, dt=dt, T0=0.0, Tf=0.0,
                     ntags_big=24, ntags_sml=24, seed=seed)
@@ -51,6 +52,7 @@ def relax_overlaps(n_big, n_sml, side, sigma_big=1.4, sigma_sml=1.0, seed=1, kap
     # below tol*sigma_sml holds once every U < kappa/4 (tol*sigma_sml)^2.
     target = 0.25 * kappa * (tol * sigma_sml) ** 2
     with Engine(p) as eng:
+        eng.set_graph_steps(graph_steps)
         eng.set_positions(x)
         done = 0
         while done < max_steps:
@@ -64,10 +66,12 @@ def relax_overlaps(n_big, n_sml, side, sigma_big=1.4, sigma_sml=1.0, seed=1, kap
         return spatial_sort(eng.positions(), n_big, side)
 
 
-def relax_batch(specs, kappa=1.0, dt=0.1, tol=1e-3, max_steps=200000, chunk=256):
+def relax_batch(specs, kappa=1.0, dt=0.1, tol=1e-3, max_steps=200000, chunk=256,
+                graph_steps=16):
     """Overlap removal for many systems in one batched engine.
 
-    specs: list of (n_big, n_sml, side, sigma_big, sigma_sml, seed).
+    specs: list of (n_big, n_sml, side, sigma_big, sigma_sml, seed);
+    graph_steps: the engine's launch mode (steps per HIP graph, 0 = eager).
     Returns a list of N_r x 3 arrays.
     """
     params = []
@@ -78,6 +82,7 @@ def relax_batch(specs, kappa=1.0, dt=0.1, tol=1e-3, max_steps=200000, chunk=256)
     sig_min = min(min(s[3], s[4]) for s in specs)
     target = 0.25 * kappa * (tol * sig_min) ** 2
     with Engine(params) as eng:
+        eng.set_graph_steps(graph_steps)
         for r, (nb, ns, side, sb, ss, seed) in enumerate(specs):
             eng.set_positions(random_positions(nb + ns, side, seed), r)
         done = 0

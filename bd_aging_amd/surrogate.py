@@ -204,9 +204,17 @@ class Surrogate:
         self.sd = None
 
     def fit(self, X, Y, epochs=60, batch=256, lr=1e-3, seed=1):
+        """Adam in four phases of epochs/4 with the rate stepped down
+        (lr, 0.3 lr, 0.1 lr, 0.03 lr); returns the per-epoch losses."""
         self.mu = X.mean(0)
         self.sd = X.std(0) + 1e-6
-        return self.net.train((X - self.mu) / self.sd, Y, epochs, batch, lr, seed)
+        Xn = (X - self.mu) / self.sd
+        phases = [max(1, epochs // 4)] * 3
+        phases.append(max(1, epochs - sum(phases)))
+        out = []
+        for k, (ep, f) in enumerate(zip(phases, (1.0, 0.3, 0.1, 0.03))):
+            out.append(self.net.train(Xn, Y, ep, batch, lr * f, seed + 7919 * k))
+        return np.concatenate(out)
 
     def predict(self, X):
         """Ensemble mean, std and the fallback mask (True: run plain BD)."""
@@ -249,22 +257,37 @@ def main(argv=None):
     ap.add_argument("--nmax", type=int, default=16000)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--out", default="surrogate_out")
+    ap.add_argument("--save-data", action="store_true",
+                    help="also write the generated data set (dataset.npz) to --out")
+    ap.add_argument("--load-data", default=None,
+                    help="train on a dataset.npz written by --save-data instead of simulating")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
 
     def log(m):
         print(m, flush=True)
 
-    xs, ys, walls, costs = [], [], [], []
-    for k in range(a.runs + a.test_runs):
-        X, Y, cost, wall = generate(R=a.replicas, n_range=(a.nmin, a.nmax), seed=1000 + k,
-                                    log=log)
-        xs.append(X)
-        ys.append(Y)
-        walls.append(wall)
-        costs.append(cost)
-    Xtr, Ytr = np.concatenate(xs[:a.runs]), np.concatenate(ys[:a.runs])
-    Xte, Yte = np.concatenate(xs[a.runs:]), np.concatenate(ys[a.runs:])
+    if a.load_data:
+        d = np.load(a.load_data)
+        Xtr, Ytr, Xte, Yte = d["Xtr"], d["Ytr"], d["Xte"], d["Yte"]
+        walls, costs = list(d["walls"]), list(d["costs"])
+        a.runs, a.test_runs = int(d["runs"]), int(d["test_runs"])
+    else:
+        xs, ys, walls, costs = [], [], [], []
+        for k in range(a.runs + a.test_runs):
+            X, Y, cost, wall = generate(R=a.replicas, n_range=(a.nmin, a.nmax), seed=1000 + k,
+                                        log=log)
+            xs.append(X)
+            ys.append(Y)
+            walls.append(wall)
+            costs.append(cost)
+        Xtr, Ytr = np.concatenate(xs[:a.runs]), np.concatenate(ys[:a.runs])
+        Xte, Yte = np.concatenate(xs[a.runs:]), np.concatenate(ys[a.runs:])
+        if a.save_data:
+            np.savez_compressed(os.path.join(a.out, "dataset.npz"), Xtr=Xtr.astype(np.float32),
+                                Ytr=Ytr.astype(np.float32), Xte=Xte.astype(np.float32),
+                                Yte=Yte.astype(np.float32), walls=np.array(walls),
+                                costs=np.array(costs), runs=a.runs, test_runs=a.test_runs)
     log(f"[surrogate] train {len(Xtr)} samples, test {len(Xte)}")
     sur = Surrogate()
     t0 = time.perf_counter()

@@ -48,7 +48,7 @@ def run_plain(steps, warmup, graph_steps=16):
     nb = ns = 16384
     side = box_side(nb, ns, 1.4, 1.0, 0.58)
     log(f"[bench] relaxing N=32768 phi=0.58 L={side:.4f}")
-    x = relax_overlaps(nb, ns, side, seed=1)
+    x = relax_overlaps(nb, ns, side, seed=1, graph_steps=graph_steps)
     p = make_params(nb, ns, side=side, dt=1e-4, T0=1.0, Tf=0.1, t0=0.0, tf=0.0,
                     first_step=-warmup, seed=777)
     eng = Engine(p)
@@ -80,7 +80,7 @@ def run_batch(steps, warmup, graph_steps=16):
         params.append(make_params(nb, ns, side=side, dt=1e-4, T0=1.0, Tf=float(T[r]), t0=0.0,
                                   tf=0.0, first_step=-warmup, seed=90000 + r))
     log(f"[bench] relaxing 256 replicas, sum N = {int(n.sum())}")
-    xs = relax_batch(specs)
+    xs = relax_batch(specs, graph_steps=graph_steps)
     eng = Engine(params)
     eng.set_graph_steps(graph_steps)
     for r, x in enumerate(xs):
@@ -131,7 +131,7 @@ def main():
                     help="steps per captured HIP graph; 0 launches every kernel eagerly "
                          "(same kernels, for kernel tracing)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="after the runs, copy a 256 MiB buffer 3 times (a plain copy kernel of "
+                    help="after the runs, copy a 1 GiB buffer 3 times (a plain copy kernel of "
                          "known traffic, the scale for FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -184,16 +184,29 @@ def main():
         copy_calibration()
 
 
-def copy_calibration(nbytes=256 << 20, reps=3):
-    import torch
-    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
-    src.uniform_()
-    dst = torch.empty_like(src)
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Device-to-device hipMemcpy of a buffer larger than the L2 and the
+    Infinity Cache (the runtime's blit kernel reads nbytes and writes nbytes
+    per call), through the HIP runtime the engine library itself links."""
+    import ctypes as C
+    from bd_aging_amd.abi import load
+    lib = load()
+    lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    lib.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    lib.hipFree.argtypes = [C.c_void_p]
+    src, dst = C.c_void_p(), C.c_void_p()
+    if lib.hipMalloc(C.byref(src), nbytes) or lib.hipMalloc(C.byref(dst), nbytes):
+        raise RuntimeError("hipMalloc failed for the copy calibration")
+    rc = lib.hipMemset(src, 1, nbytes)
     for _ in range(reps):
-        dst.copy_(src)
-    torch.cuda.synchronize()
+        rc = rc or lib.hipMemcpy(dst, src, nbytes, 3)  # hipMemcpyDeviceToDevice
+    rc = rc or lib.hipDeviceSynchronize()
+    lib.hipFree(src)
+    lib.hipFree(dst)
+    if rc:
+        raise RuntimeError(f"copy calibration failed (hip error {rc})")
     log(f"[bench] copy calibration: {reps} x {nbytes} B read + {nbytes} B written")
-
 
 if __name__ == "__main__":
     main()
