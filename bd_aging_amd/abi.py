@@ -92,6 +92,8 @@ SIGNATURES = {
     "bd_get_force_order": (C.c_int, [_P, _I]),
     "bd_set_graph_steps": (C.c_int, [_P, C.c_int32]),
     "bd_get_graph_steps": (C.c_int, [_P, _I]),
+    "bd_set_defer_depth": (C.c_int, [_P, C.c_int32]),
+    "bd_get_defer_depth": (C.c_int, [_P, _I]),
     "bd_set_brick": (C.c_int, [_P, C.c_int32]),
     "bd_get_brick": (C.c_int, [_P, _I]),
     "bd_kernel_times": (C.c_int, [_P, C.c_int32, _D]),

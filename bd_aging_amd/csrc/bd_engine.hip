@@ -11,9 +11,15 @@
 //   vir, pot         [N]
 //   nocup            int per cell   atomic insertion counters (zero between steps)
 //   cnt              int per cell   occupancy of the last binning
-//   ins              int per slot   insertion-order particle ids (ntags slots per cell)
-//   srt              int per slot   the same ids in ascending order (reference layout)
-//   cpos             double4 per slot  x, y, z, replica-local index of srt[slot]
+//   ins              int per slot   insertion-order particle ids (ntags slots per
+//                                   cell, cell index x + n (y + n z))
+//   cpos             double4 per slot  x, y, z, replica-local index, in ROW
+//                    layout: row (x, y) owns the n * ntags slots from
+//                    (x + n y) n ntags; its particles are packed from the
+//                    start in (z, ascending index) order, i.e. the order in
+//                    which the reference's innermost k_del loop meets them
+//   rs               int per (row, z) + 1  slot where cell z of the row starts
+//                    (rs[row (n+1) + n] = end of the row), grid-relative
 //   ring             u64 [250][lanes_total]  rannew64 lagged-Fibonacci rings
 //   lcg              u64 [lanes_total]
 //   draws            u64 per replica  draws taken by every lane of the replica
@@ -24,9 +30,9 @@
 //   k_update_bin  Ermak-McCammon update fused with the rannew64 Gaussian stream
 //                 and the atomic cell insertion (update_ermak.cu:91-160 +
 //                 cell_locate.cu:155-209)
-//   k_cells       per-cell canonical ordering (ascending index, the order of
-//                 cell_locate_hst), gather of the cell-ordered positions,
-//                 counter reset and the step/draw counters
+//   k_cells       per-row packing of the cells in canonical order (ascending
+//                 index within a cell, the order of cell_locate_hst) with
+//                 their positions, counter reset and the step/draw counters
 //   k_forces      per-particle WCA forces over the same-species then the
 //                 cross-species stencil in the reference's traversal order
 //                 (get_forces.cu:297-558), written (not accumulated)
@@ -60,6 +66,7 @@ struct RepDev {
   int32_t ntags[2];
   int32_t cell_off[2];
   int32_t tag_off[2];
+  int32_t rs_off[2];        // row-start tables, ncell^2 * (ncell + 1) ints per grid
   int32_t nrange_same[2];   // bb, ss
   int32_t nrange_cross[2];  // big particle -> small grid (bs), small -> big (sb)
   int32_t potential, temp_protocol;
@@ -255,16 +262,21 @@ __global__ void __launch_bounds__(kBlk)
   if (bad) atomicOr(&flags[t.x], kFlagNonFinite);
 }
 
-// Per-cell canonical order: the ids inserted by atomics are ranked by value
-// (ascending, the serial order of cell_locate_hst, cell_locate.cu:72-101),
-// written to srt, and their positions gathered into cpos.  The insertion
-// counter is reset for the next binning.  With advance != 0 the first thread
-// also advances the step index and the per-replica draw counters.
+// Row packing, one wavefront per row (x, y) of a grid: lane z reads the
+// occupancy of cell (x, y, z), a wavefront prefix sum places the cells one
+// after the other from the row's first slot, and each lane writes its
+// cell's particles ranked by id (ascending, the serial order of
+// cell_locate_hst, cell_locate.cu:72-101) with their positions.  Rows longer
+// than 64 cells are packed in chunks of 64.  The insertion counters are
+// reset for the next binning.  With advance != 0 the first thread also
+// advances the step index and the per-replica draw counters.
+constexpr int kRowsPerBlock = kBlk / 64;
+
 __global__ void __launch_bounds__(kBlk)
     k_cells(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
             const double* __restrict__ px, const double* __restrict__ py,
             const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
-            const int* __restrict__ ins, int* __restrict__ srt, double4* __restrict__ cpos,
+            const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
             int* __restrict__ flags, int64_t* __restrict__ step,
             uint64_t* __restrict__ draws, int n_reps, int advance) {
   if (advance && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -273,27 +285,52 @@ __global__ void __launch_bounds__(kBlk)
   }
   const int4 t = tbl[blockIdx.x];
   if (t.x < 0) return;
-  const int c = t.y + (int)threadIdx.x;
-  if (c >= t.z) return;
+  const int row = t.y + (int)threadIdx.x / 64;  // x + n y
+  if (row >= t.z) return;                         // whole wavefront
+  const int lane = (int)threadIdx.x % 64;
   const int g = t.w;
   const RepDev& R = reps[t.x];
-  const int gc = R.cell_off[g] + c;
-  const int n = nocup[gc];
+  const int n = R.ncell[g];
   const int ntags = R.ntags[g];
-  const int m = n < ntags ? n : ntags;
-  const size_t base = (size_t)R.tag_off[g] + (size_t)c * ntags;
   const int sp_off = g == 0 ? 0 : R.nb;
-  for (int k = 0; k < m; k++) {
-    const int idk = ins[base + k];
-    int rank = 0;
-    for (int j = 0; j < m; j++) rank += ins[base + j] < idk ? 1 : 0;
-    srt[base + rank] = idk;
-    const int p = R.p_off + sp_off + idk;
-    cpos[base + rank] = make_double4(px[p], py[p], pz[p], (double)(sp_off + idk));
+  const int row_base = row * n * ntags;  // grid-relative slot of the row
+  int* rrow = rs + R.rs_off[g] + row * (n + 1);
+  double4* cp = cpos + R.tag_off[g];
+  const int* ins_g = ins + R.tag_off[g];
+  int carry = 0;
+  bool over = false;
+  for (int z0 = 0; z0 < n; z0 += 64) {
+    const int z = z0 + lane;
+    const bool live = z < n;
+    const int c = row + n * n * z;  // x + n (y + n z)
+    int occ = 0;
+    if (live) occ = nocup[R.cell_off[g] + c];
+    const int m = occ < ntags ? occ : ntags;
+    int incl = m;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    const int start = row_base + carry + incl - m;
+    if (live) {
+      rrow[z] = start;
+      const int* slots = ins_g + (size_t)c * ntags;
+      for (int k = 0; k < m; k++) {
+        const int idk = slots[k];
+        int rank = 0;
+        for (int j = 0; j < m; j++) rank += slots[j] < idk ? 1 : 0;
+        const int p = R.p_off + sp_off + idk;
+        cp[start + rank] = make_double4(px[p], py[p], pz[p], (double)(sp_off + idk));
+      }
+      cnt[R.cell_off[g] + c] = occ;
+      nocup[R.cell_off[g] + c] = 0;
+      over = over || occ >= ntags;
+    }
+    carry += __shfl(incl, 63, 64);
   }
-  cnt[gc] = n;
-  nocup[gc] = 0;
-  if (n >= ntags) atomicOr(&flags[t.x], kFlagOverflow);
+  if (lane == 0) rrow[n] = row_base + carry;
+  if (over) atomicOr(&flags[t.x], kFlagOverflow);
 }
 
 // Distance along one axis from a point to the cell at offset d of its home
@@ -318,18 +355,17 @@ __device__ inline double axis_gap(int d, double cs, double u, double L) {
 template <int POT, bool NEIGH>
 __device__ inline void scan_grid(const RepDev& R, int g, int nrange, double sigma, double rc2,
                                  int self, double x, double y, double z,
-                                 const int* __restrict__ cnt, const double4* __restrict__ cpos,
+                                 const int* __restrict__ rs, const double4* __restrict__ cpos,
                                  double& fx, double& fy, double& fz, double& vir, double& pot,
                                  int* nlist, int& ncount, int maxn) {
   int ii, jj, kk;
   if (!cell_coords(R, g, x, y, z, ii, jj, kk)) return;
   const int n = R.ncell[g];
-  const int ntags = R.ntags[g];
   const double side = R.side, side_inv = R.side_inv;
   const double cs = R.cell_side[g];
   const double ux = x - (double)ii * cs, uy = y - (double)jj * cs, uz = z - (double)kk * cs;
   const double lim = rc2 + 1e-9;
-  const int* cnt_g = cnt + R.cell_off[g];
+  const int* rs_g = rs + R.rs_off[g];
   const double4* cp_g = cpos + R.tag_off[g];
   for (int di = -nrange; di <= nrange; di++) {
     const double gx = axis_gap(di, cs, ux, side);
@@ -348,10 +384,10 @@ __device__ inline void scan_grid(const RepDev& R, int g, int nrange, double sigm
         if (gxy2 + gz * gz > lim) continue;
         int c = kk + dk;
         c = c < 0 ? c + n : (c >= n ? c - n : c);
-        const int ci = a + n * (b + n * c);
-        int occ = cnt_g[ci];
-        occ = occ < ntags ? occ : ntags;
-        const double4* cell = cp_g + (size_t)ci * ntags;
+        const int ri = (a + n * b) * (n + 1) + c;
+        const int s0 = rs_g[ri];
+        const int occ = rs_g[ri + 1] - s0;
+        const double4* cell = cp_g + s0;
         for (int s = 0; s < occ; s++) {
           const double4 q = cell[s];
           if (self >= 0 && q.w == (double)self) continue;
@@ -400,19 +436,19 @@ __device__ inline void scan_grid(const RepDev& R, int g, int nrange, double sigm
 // scan_grid.  ACC maps accessor-frame cell coordinates to a slot range.
 // ---------------------------------------------------------------------------
 struct GlobalCells {
-  const int* cnt;
+  const int* rs;  // the grid's row-start table
   const double4* cp;
-  int n, ntags;
+  int n;
   __device__ int index(int a, int b, int c) const {
     a = a < 0 ? a + n : (a >= n ? a - n : a);
     b = b < 0 ? b + n : (b >= n ? b - n : b);
     c = c < 0 ? c + n : (c >= n ? c - n : c);
-    return a + n * (b + n * c);
+    return (a + n * b) * (n + 1) + c;
   }
   __device__ void cell(int idx, int& occ, const double4*& ptr) const {
-    const int o = cnt[idx];
-    occ = o < ntags ? o : ntags;
-    ptr = cp + (size_t)idx * ntags;
+    const int s0 = rs[idx];
+    occ = rs[idx + 1] - s0;
+    ptr = cp + s0;
   }
 };
 
@@ -535,13 +571,173 @@ __device__ inline void scan_fast_nr(int nr, const RepDev& R, int n, double cs, d
                              pot, sub, G, cbase);
 }
 
+// One pair term inside the cutoff (get_forces.cu:396-411): potential, force
+// along the separation, accumulations.
+template <int POT>
+__device__ inline void pair_add(const RepDev& R, double sigma, double dist2, double dx, double dy,
+                                double dz, double& fx, double& fy, double& fz, double& vir,
+                                double& pot) {
+  const double dist_inv = sqrt(1.0 / dist2);
+  double nf;
+  bd::potential<POT>(R.kappa, sigma, dist_inv, pot, nf);
+  const double qx = nf * dist_inv * dx;
+  const double qy = nf * dist_inv * dy;
+  const double qz = nf * dist_inv * dz;
+  fx += qx;
+  fy += qy;
+  fz += qz;
+  vir -= 0.5 * (dx * qx + dy * qy + dz * qz);
+}
+
+// Deferred pair terms: the walk records each partner inside the cutoff as
+// slot | image codes (2 bits per axis: 0 none, 1 -L, 2 +L) in a per-thread
+// LDS column; drain() evaluates the recorded terms in recording order,
+// recomputing the separation with the same operations.  The potential is
+// then evaluated by all lanes together on interacting pairs only, instead of
+// by the whole wavefront whenever one lane meets a partner.
+constexpr unsigned int kSlotBits = 26;
+constexpr unsigned int kSlotMask = (1u << kSlotBits) - 1u;
+
+__device__ inline unsigned int image_code(double sh) {
+  return sh == 0.0 ? 0u : (sh < 0.0 ? 1u : 2u);
+}
+
+template <int POT>
+__device__ inline void drain(const RepDev& R, double sigma, double x, double y, double z,
+                             const double4* __restrict__ cp, const unsigned int* buf, int nd,
+                             double& fx, double& fy, double& fz, double& vir, double& pot) {
+  const double L = R.side;
+  for (int e = 0; e < nd; e++) {
+    const unsigned int w = buf[e * kBlk];
+    const double4 q = cp[w & kSlotMask];
+    const unsigned int cx = (w >> kSlotBits) & 3u, cy = (w >> (kSlotBits + 2)) & 3u,
+                       cz = w >> (kSlotBits + 4);
+    double dx = q.x - x;
+    double dy = q.y - y;
+    double dz = q.z - z;
+    if (cx != 0u) dx = dx + (cx == 1u ? -L : L);
+    if (cy != 0u) dy = dy + (cy == 1u ? -L : L);
+    if (cz != 0u) dz = dz + (cz == 1u ? -L : L);
+    const double dist2 = dx * dx + dy * dy + dz * dz;
+    pair_add<POT>(R, sigma, dist2, dx, dy, dz, fx, fy, fz, vir, pot);
+  }
+}
+
+// Row-range stencil walk, the default force path.  For every stencil row
+// (di, dj) that passes the slab-gap test, the cells k_del = klo..khi that
+// can hold a partner are consecutive z cells of one packed row, i.e. one
+// slot range, or two when the run crosses the periodic boundary (taken in
+// k_del order: the wrapped low cells first, or the wrapped high cells
+// last).  The ranges are walked as one loop: same pairs, same order and the
+// same IEEE operations as scan_fast, with one loop per row instead of one
+// per cell.  DEPTH > 0 defers the pair terms through `buf` (DEPTH entries,
+// stride kBlk), draining whenever it fills and at the end.
+template <int POT, int NR, int DEPTH>
+__device__ inline void scan_rows(const RepDev& R, int n, double cs, double sigma, double rc2,
+                                 int self, double x, double y, double z, const int hw[3],
+                                 const int* __restrict__ rs, const double4* __restrict__ cp,
+                                 unsigned int* buf, double& fx, double& fy, double& fz,
+                                 double& vir, double& pot) {
+  const double L = R.side;
+  const double lim = rc2 + 1e-9;
+  const double ux = x - (double)hw[0] * cs, uy = y - (double)hw[1] * cs,
+               uz = z - (double)hw[2] * cs;
+  double g2lo[NR], g2hi[NR];  // squared z gaps of k_del = -(k+1) and +(k+1)
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    const double lo = slab_gap(-(k + 1), cs, uz);
+    const double hi = slab_gap(k + 1, cs, uz);
+    g2lo[k] = lo * lo;
+    g2hi[k] = hi * hi;
+  }
+  int nd = 0;
+  for (int di = -NR; di <= NR; di++) {
+    const double gx = slab_gap(di, cs, ux);
+    const double gx2 = gx * gx;
+    if (gx2 > lim) continue;
+    const double shx = image_shift(hw[0] + di, n, L);
+    int a = hw[0] + di;
+    a = a < 0 ? a + n : (a >= n ? a - n : a);
+    for (int dj = -NR; dj <= NR; dj++) {
+      const double gy = slab_gap(dj, cs, uy);
+      const double gxy2 = gx2 + gy * gy;
+      if (gxy2 > lim) continue;
+      const double shy = image_shift(hw[1] + dj, n, L);
+      int b = hw[1] + dj;
+      b = b < 0 ? b + n : (b >= n ? b - n : b);
+      // the gaps grow with |k_del|, so the passing offsets are contiguous
+      int klo = 0, khi = 0;
+#pragma unroll
+      for (int k = 0; k < NR; k++) {
+        if (gxy2 + g2lo[k] <= lim) klo = -(k + 1);
+        if (gxy2 + g2hi[k] <= lim) khi = k + 1;
+      }
+      const int* rrow = rs + (a + n * b) * (n + 1);
+      const int z0 = hw[2] + klo, z1 = hw[2] + khi;
+      int s0a, s1a, s0b = 0, s1b = 0;
+      double sha = 0.0, shb = 0.0;
+      if (z0 < 0) {  // cells z0+n .. n-1 seen through the -L image, then 0 .. z1
+        s0a = rrow[z0 + n];
+        s1a = rrow[n];
+        sha = -L;
+        s0b = rrow[0];
+        s1b = rrow[z1 + 1];
+      } else if (z1 >= n) {  // cells z0 .. n-1, then 0 .. z1-n through the +L image
+        s0a = rrow[z0];
+        s1a = rrow[n];
+        s0b = rrow[0];
+        s1b = rrow[z1 - n + 1];
+        shb = L;
+      } else {
+        s0a = rrow[z0];
+        s1a = rrow[z1 + 1];
+      }
+      const int na = s1a - s0a;
+      const int tot = na + (s1b - s0b);
+      const bool center = self >= 0 && di == 0 && dj == 0;
+      const unsigned int cxy =
+          (image_code(shx) << kSlotBits) | (image_code(shy) << (kSlotBits + 2));
+      for (int t = 0; t < tot; t++) {
+        const bool in_a = t < na;
+        const int sl = in_a ? s0a + t : s0b + (t - na);
+        const double4 q = cp[sl];
+        if (center && q.w == (double)self) continue;
+        const double shz = in_a ? sha : shb;
+        double dx = q.x - x;
+        double dy = q.y - y;
+        double dz = q.z - z;
+        if (shx != 0.0) dx = dx + shx;
+        if (shy != 0.0) dy = dy + shy;
+        if (shz != 0.0) dz = dz + shz;
+        const double dist2 = dx * dx + dy * dy + dz * dz;
+        if (dist2 < rc2) {
+          if (DEPTH == 0) {
+            pair_add<POT>(R, sigma, dist2, dx, dy, dz, fx, fy, fz, vir, pot);
+          } else {
+            buf[nd * kBlk] = (unsigned int)sl | cxy | (image_code(shz) << (kSlotBits + 4));
+            if (++nd == DEPTH) {
+              drain<POT>(R, sigma, x, y, z, cp, buf, nd, fx, fy, fz, vir, pot);
+              nd = 0;
+            }
+          }
+        }
+      }
+    }
+  }
+  if (DEPTH > 0) drain<POT>(R, sigma, x, y, z, cp, buf, nd, fx, fy, fz, vir, pot);
+}
+
 // Both stencils of replica-local particle i (same species, then the other
 // species, main.cu:576-585) from global memory, as one rolled pass loop.
-template <int POT, bool SPLIT>
+// SPLIT walks cells (scan_fast) so that G lanes can share the stencil; the
+// single-lane walk uses row ranges (scan_rows), deferring pair terms through
+// `buf` when DEPTH > 0.
+template <int POT, bool SPLIT, int DEPTH = 0>
 __device__ inline void forces_global(const RepDev& R, int i, double x, double y, double z,
-                                     const int* __restrict__ cnt,
+                                     const int* __restrict__ rs,
                                      const double4* __restrict__ cpos, double& fx, double& fy,
-                                     double& fz, double& v, double& u, int sub = 0, int G = 1) {
+                                     double& fz, double& v, double& u, int sub = 0, int G = 1,
+                                     unsigned int* buf = nullptr) {
   const int g = i < R.nb ? 0 : 1;
   int cbase = 0;
 #pragma unroll
@@ -556,15 +752,24 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
       int hg[3];
       if (!cell_coords(R, gg, x, y, z, hg[0], hg[1], hg[2])) return;
       const int n = gg == 0 ? R.ncell[0] : R.ncell[1];
-      const GlobalCells acc{cnt + (gg == 0 ? R.cell_off[0] : R.cell_off[1]),
-                            cpos + (gg == 0 ? R.tag_off[0] : R.tag_off[1]), n,
-                            gg == 0 ? R.ntags[0] : R.ntags[1]};
-      scan_fast_nr<POT, SPLIT>(nr, R, n, gg == 0 ? R.cell_side[0] : R.cell_side[1], sigma, rc2,
-                               self, x, y, z, hg, hg, acc, fx, fy, fz, v, u, sub, G, cbase);
-      cbase += (2 * nr + 1) * (2 * nr + 1) * (2 * nr + 1);
+      const double cs = gg == 0 ? R.cell_side[0] : R.cell_side[1];
+      const int* rs_g = rs + (gg == 0 ? R.rs_off[0] : R.rs_off[1]);
+      const double4* cp_g = cpos + (gg == 0 ? R.tag_off[0] : R.tag_off[1]);
+      if (SPLIT) {
+        const GlobalCells acc{rs_g, cp_g, n};
+        scan_fast_nr<POT, true>(nr, R, n, cs, sigma, rc2, self, x, y, z, hg, hg, acc, fx, fy,
+                                fz, v, u, sub, G, cbase);
+        cbase += (2 * nr + 1) * (2 * nr + 1) * (2 * nr + 1);
+      } else if (nr == 1) {
+        scan_rows<POT, 1, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, buf, fx,
+                                 fy, fz, v, u);
+      } else {
+        scan_rows<POT, 2, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, buf, fx,
+                                 fy, fz, v, u);
+      }
     } else {
       int nc = 0;
-      scan_grid<POT, false>(R, gg, nr, sigma, rc2, self, x, y, z, cnt, cpos, fx, fy, fz, v, u,
+      scan_grid<POT, false>(R, gg, nr, sigma, rc2, self, x, y, z, rs, cpos, fx, fy, fz, v, u,
                             nullptr, nc, 0);
     }
   }
@@ -573,15 +778,17 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
 // Forces, virials and potentials of every particle: same-species stencil
 // then cross-species stencil (main.cu:576-585), accumulators start at zero.
 // NEIGH records the interacting partners in traversal order (test hook).
-template <int POT, bool NEIGH>
+// DEPTH: per-thread LDS entries for deferred pair terms (0 = direct).
+template <int POT, bool NEIGH, int DEPTH>
 __global__ void __launch_bounds__(kBlk)
     k_forces(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
              const double* __restrict__ px, const double* __restrict__ py,
-             const double* __restrict__ pz, const int* __restrict__ cnt,
+             const double* __restrict__ pz, const int* __restrict__ rs,
              const double4* __restrict__ cpos, double* __restrict__ ffx,
              double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
              double* __restrict__ pot, int* __restrict__ ncounts, int* __restrict__ nlist,
              int maxn) {
+  __shared__ unsigned int dbuf[DEPTH > 0 ? DEPTH * kBlk : 1];
   const int4 t = tbl[blockIdx.x];
   if (t.x < 0) return;
   const int i = t.y + (int)threadIdx.x;
@@ -594,13 +801,14 @@ __global__ void __launch_bounds__(kBlk)
   if (NEIGH) {
     int nc = 0;
     int* nl = nlist + (size_t)i * maxn;
-    scan_grid<POT, true>(R, g, R.nrange_same[g], R.sigma[g], R.rc2_same[g], i, x, y, z, cnt,
+    scan_grid<POT, true>(R, g, R.nrange_same[g], R.sigma[g], R.rc2_same[g], i, x, y, z, rs,
                          cpos, fx, fy, fz, v, u, nl, nc, maxn);
-    scan_grid<POT, true>(R, 1 - g, R.nrange_cross[g], R.sigma_bs, R.rc2_bs, -1, x, y, z, cnt,
+    scan_grid<POT, true>(R, 1 - g, R.nrange_cross[g], R.sigma_bs, R.rc2_bs, -1, x, y, z, rs,
                          cpos, fx, fy, fz, v, u, nl, nc, maxn);
     ncounts[i] = nc;
   } else {
-    forces_global<POT, false>(R, i, x, y, z, cnt, cpos, fx, fy, fz, v, u);
+    forces_global<POT, false, DEPTH>(R, i, x, y, z, rs, cpos, fx, fy, fz, v, u, 0, 1,
+                                     dbuf + threadIdx.x);
   }
   ffx[p] = fx;
   ffy[p] = fy;
@@ -620,7 +828,7 @@ template <int POT, int G>
 __global__ void __launch_bounds__(kBlk)
     k_forces_split(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
                    const double* __restrict__ px, const double* __restrict__ py,
-                   const double* __restrict__ pz, const int* __restrict__ cnt,
+                   const double* __restrict__ pz, const int* __restrict__ rs,
                    const double4* __restrict__ cpos, double* __restrict__ ffx,
                    double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
                    double* __restrict__ pot) {
@@ -633,7 +841,7 @@ __global__ void __launch_bounds__(kBlk)
   double fx = 0.0, fy = 0.0, fz = 0.0, v = 0.0, u = 0.0;
   if (live) {
     const int p = R.p_off + i;
-    forces_global<POT, true>(R, i, px[p], py[p], pz[p], cnt, cpos, fx, fy, fz, v, u, sub, G);
+    forces_global<POT, true>(R, i, px[p], py[p], pz[p], rs, cpos, fx, fy, fz, v, u, sub, G);
   }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) {
@@ -663,7 +871,8 @@ __global__ void __launch_bounds__(kBlk)
 // the brick: the big cells of the brick widened by max(nrange_bb, nrange_sb)
 // and the small cells a home particle can occupy widened by
 // max(nrange_ss, nrange_bs).  Cells are addressed by unwrapped coordinates
-// relative to the box and staged row by row (x runs, coalesced).  Home
+// relative to the box and staged as z runs (each a contiguous slot range of
+// a packed row, or two where it wraps).  Home
 // particles are listed in cell order, then each thread walks one particle's
 // two stencils from LDS (scan_fast), so the forces are bit-identical to the
 // global-memory kernels.  A brick that would stage more than CAP positions
@@ -713,7 +922,7 @@ template <int POT, int CAP>
 __global__ void __launch_bounds__(kBrickThreads)
     k_forces_brick(const RepDev* __restrict__ reps, const int4* __restrict__ btbl, int B,
                    const double* __restrict__ px, const double* __restrict__ py,
-                   const double* __restrict__ pz, const int* __restrict__ cnt,
+                   const double* __restrict__ pz, const int* __restrict__ rs,
                    const double4* __restrict__ cpos,
                    double* __restrict__ ffx, double* __restrict__ ffy, double* __restrict__ ffz,
                    double* __restrict__ vir, double* __restrict__ pot, int* __restrict__ flags) {
@@ -728,7 +937,7 @@ __global__ void __launch_bounds__(kBrickThreads)
   const RepDev& R = reps[t.x];
   const int b0[3] = {t.y, t.z, t.w};
   const int rb = max(R.nrange_same[0], R.nrange_cross[1]);
-  const int rs = max(R.nrange_same[1], R.nrange_cross[0]);
+  const int rsm = max(R.nrange_same[1], R.nrange_cross[0]);
   const int nbg = R.ncell[0], nsg = R.ncell[1];
   BrickBox bb, sb;
   int bext[3], shl[3], shh[3];
@@ -741,35 +950,34 @@ __global__ void __launch_bounds__(kBrickThreads)
     const int h = (int)floor((double)(b0[a] + bext[a]) * R.cell_side[0] * R.cell_inv[1] + 1e-9);
     shl[a] = l;
     shh[a] = min(h, l + nsg - 1);
-    sb.lo[a] = l - rs;
-    sb.dim[a] = h - l + 2 * rs + 1;
+    sb.lo[a] = l - rsm;
+    sb.dim[a] = h - l + 2 * rsm + 1;
   }
   const int nbig = bb.dim[0] * bb.dim[1] * bb.dim[2];
   const int ncells = nbig + sb.dim[0] * sb.dim[1] * sb.dim[2];
-  const int nbr = bb.dim[1] * bb.dim[2];
-  const int nrows = nbr + sb.dim[1] * sb.dim[2];
+  const int nbr = bb.dim[0] * bb.dim[1];
+  const int nrows = nbr + sb.dim[0] * sb.dim[1];
   if (ncells > kBrickMaxCells || nrows > kBrickMaxRows) {  // excluded on the host
     if (tid == 0) atomicOr(&flags[t.x], kFlagBrick);
     return;
   }
-  // 1. occupancy of every staged cell, one x-run (row) per thread
+  // 1. occupancy of every staged cell, one z-run (packed row) per thread
   for (int r = tid; r < nrows; r += kBrickThreads) {
     const int g = r < nbr ? 0 : 1;
     const BrickBox& bx = g == 0 ? bb : sb;
     const int n = g == 0 ? nbg : nsg;
-    const int ntags = g == 0 ? R.ntags[0] : R.ntags[1];
     const int rr = g == 0 ? r : r - nbr;
-    const int ry = rr % bx.dim[1], rz = rr / bx.dim[1];
-    const int* crow = cnt + (g == 0 ? R.cell_off[0] : R.cell_off[1]) +
-                      n * (wrap_cell(bx.lo[1] + ry, n) + n * wrap_cell(bx.lo[2] + rz, n));
-    const int cbase = (g == 0 ? 0 : nbig) + bx.dim[0] * rr;
-    int wx = wrap_cell(bx.lo[0], n), sum = 0;
-    for (int rx = 0; rx < bx.dim[0]; rx++) {
-      const int o = crow[wx];
-      const int occ = o < ntags ? o : ntags;
-      tbl[cbase + rx] = (unsigned int)occ;
+    const int rx = rr % bx.dim[0], ry = rr / bx.dim[0];
+    const int* rrow = rs + (g == 0 ? R.rs_off[0] : R.rs_off[1]) +
+                      (wrap_cell(bx.lo[0] + rx, n) + n * wrap_cell(bx.lo[1] + ry, n)) * (n + 1);
+    const int cbase = (g == 0 ? 0 : nbig) + rx + bx.dim[0] * ry;
+    const int cstride = bx.dim[0] * bx.dim[1];
+    int wz = wrap_cell(bx.lo[2], n), sum = 0;
+    for (int rz = 0; rz < bx.dim[2]; rz++) {
+      const int occ = rrow[wz + 1] - rrow[wz];
+      tbl[cbase + cstride * rz] = (unsigned int)occ;
       sum += occ;
-      wx = wx + 1 == n ? 0 : wx + 1;
+      wz = wz + 1 == n ? 0 : wz + 1;
     }
     rows[r] = sum;
   }
@@ -777,27 +985,29 @@ __global__ void __launch_bounds__(kBrickThreads)
   // 2. row offsets into the staged position array
   const int total = block_scan(rows, nrows, part);
   const bool staged = total <= CAP;
-  // 3. cell offsets and the gather of positions (row by row)
+  // 3. cell offsets and the gather of positions (run by run)
   for (int r = tid; r < nrows; r += kBrickThreads) {
     const int g = r < nbr ? 0 : 1;
     const BrickBox& bx = g == 0 ? bb : sb;
     const int n = g == 0 ? nbg : nsg;
-    const int ntags = g == 0 ? R.ntags[0] : R.ntags[1];
     const int rr = g == 0 ? r : r - nbr;
-    const int ry = rr % bx.dim[1], rz = rr / bx.dim[1];
-    const int rowcell = n * (wrap_cell(bx.lo[1] + ry, n) + n * wrap_cell(bx.lo[2] + rz, n));
+    const int rx = rr % bx.dim[0], ry = rr / bx.dim[0];
+    const int* rrow = rs + (g == 0 ? R.rs_off[0] : R.rs_off[1]) +
+                      (wrap_cell(bx.lo[0] + rx, n) + n * wrap_cell(bx.lo[1] + ry, n)) * (n + 1);
     const double4* src = cpos + (g == 0 ? R.tag_off[0] : R.tag_off[1]);
-    const int cbase = (g == 0 ? 0 : nbig) + bx.dim[0] * rr;
-    int wx = wrap_cell(bx.lo[0], n), start = rows[r];
-    for (int rx = 0; rx < bx.dim[0]; rx++) {
-      const int occ = (int)tbl[cbase + rx];
-      tbl[cbase + rx] = ((unsigned int)start << 6) | (unsigned int)occ;
+    const int cbase = (g == 0 ? 0 : nbig) + rx + bx.dim[0] * ry;
+    const int cstride = bx.dim[0] * bx.dim[1];
+    int wz = wrap_cell(bx.lo[2], n), start = rows[r];
+    for (int rz = 0; rz < bx.dim[2]; rz++) {
+      const int idx = cbase + cstride * rz;
+      const int occ = (int)tbl[idx];
+      tbl[idx] = ((unsigned int)start << 6) | (unsigned int)occ;
       if (staged) {
-        const double4* cs = src + (size_t)(rowcell + wx) * ntags;
+        const double4* cs = src + rrow[wz];
         for (int s = 0; s < occ; s++) spos[start + s] = cs[s];
       }
       start += occ;
-      wx = wx + 1 == n ? 0 : wx + 1;
+      wz = wz + 1 == n ? 0 : wz + 1;
     }
   }
   __syncthreads();
@@ -806,8 +1016,8 @@ __global__ void __launch_bounds__(kBrickThreads)
   // 4. home particles in cell order: big rows of the brick, then small rows
   //    holding particles whose big cell is in the brick (filtered).  The
   //    list holds LDS slots (staged) or replica-local indices (not staged).
-  const GlobalCells gb{cnt + R.cell_off[0], cpos + R.tag_off[0], nbg, R.ntags[0]};
-  const GlobalCells gs{cnt + R.cell_off[1], cpos + R.tag_off[1], nsg, R.ntags[1]};
+  const GlobalCells gb{rs + R.rs_off[0], cpos + R.tag_off[0], nbg};
+  const GlobalCells gs{rs + R.rs_off[1], cpos + R.tag_off[1], nsg};
   const int hb = bext[1] * bext[2];
   const int sh1 = shh[1] - shl[1] + 1, sh2 = shh[2] - shl[2] + 1;
   const int nhr = hb + sh1 * sh2;
@@ -909,7 +1119,7 @@ __global__ void __launch_bounds__(kBrickThreads)
     } else {
       i = home[h];
       const int p = R.p_off + i;
-      forces_global<POT, false>(R, i, px[p], py[p], pz[p], cnt, cpos, fx, fy, fz, v, u);
+      forces_global<POT, false>(R, i, px[p], py[p], pz[p], rs, cpos, fx, fy, fz, v, u);
     }
     const int p = R.p_off + i;
     ffx[p] = fx;
@@ -1125,7 +1335,7 @@ struct bd_sys {
   std::vector<bd_geometry> geom;
   std::vector<RepDev> reps;
   std::vector<bool> have_pos;
-  int n_total = 0, lanes_total = 0, cells_total = 0;
+  int n_total = 0, lanes_total = 0, cells_total = 0, rs_total = 0;
   size_t tags_total = 0;
   int64_t ni = 0;
   // launch tables
@@ -1137,7 +1347,7 @@ struct bd_sys {
   double *ix = nullptr, *iy = nullptr, *iz = nullptr;
   double *fx = nullptr, *fy = nullptr, *fz = nullptr;
   double *vir = nullptr, *pot = nullptr;
-  int *nocup = nullptr, *cnt = nullptr, *ins = nullptr, *srt = nullptr;
+  int *nocup = nullptr, *cnt = nullptr, *ins = nullptr, *rs = nullptr;
   double4* cpos = nullptr;
   uint64_t *ring = nullptr, *lcg = nullptr, *draws = nullptr;
   int64_t* step = nullptr;
@@ -1154,6 +1364,9 @@ struct bd_sys {
   int4* tbl_brick = nullptr;
   // force summation order: 1 = reference (bit-exact), G = 4 or 8 = split
   int order = 1;
+  // deferred pair terms of the row walk: 0 (direct), 16 or 32 LDS entries
+  int defer = 16;
+  bool defer_ok = true;  // every grid's slot index fits the 26-bit record
   std::vector<int4> tbl_split_h;
   int4* tbl_split = nullptr;
   // waiting-time origins: BD_MAX_ORIGINS slots of raw positions (x, y, z)
@@ -1180,29 +1393,39 @@ void launch_update(bd_sys* s, int do_bin) {
 void launch_cells(bd_sys* s, int advance) {
   const int nb = (int)s->tbl_cell_h.size();
   k_cells<<<nb, kBlk, 0, s->stream>>>(s->reps_d, s->tbl_cell, s->px, s->py, s->pz, s->nocup,
-                                      s->cnt, s->ins, s->srt, s->cpos, s->flags, s->step,
+                                      s->cnt, s->ins, s->rs, s->cpos, s->flags, s->step,
                                       s->draws, s->R, advance);
+}
+
+template <bool NEIGH, int DEPTH>
+void launch_forces_k(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl, int nblk) {
+  switch (s->pot_kind) {
+    case BD_POT_WCA:
+      k_forces<BD_POT_WCA, NEIGH, DEPTH><<<nblk, kBlk, 0, s->stream>>>(
+          s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
+          s->pot, ncounts, nlist, maxn);
+      break;
+    case BD_POT_WCA_MODIFIED:
+      k_forces<BD_POT_WCA_MODIFIED, NEIGH, DEPTH><<<nblk, kBlk, 0, s->stream>>>(
+          s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
+          s->pot, ncounts, nlist, maxn);
+      break;
+    default:
+      k_forces<BD_POT_HARMONIC, NEIGH, DEPTH><<<nblk, kBlk, 0, s->stream>>>(
+          s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
+          s->pot, ncounts, nlist, maxn);
+      break;
+  }
 }
 
 template <bool NEIGH>
 void launch_forces_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl, int nblk) {
-  switch (s->pot_kind) {
-    case BD_POT_WCA:
-      k_forces<BD_POT_WCA, NEIGH><<<nblk, kBlk, 0, s->stream>>>(
-          s->reps_d, tbl, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot, ncounts, nlist, maxn);
-      break;
-    case BD_POT_WCA_MODIFIED:
-      k_forces<BD_POT_WCA_MODIFIED, NEIGH><<<nblk, kBlk, 0, s->stream>>>(
-          s->reps_d, tbl, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot, ncounts, nlist, maxn);
-      break;
-    default:
-      k_forces<BD_POT_HARMONIC, NEIGH><<<nblk, kBlk, 0, s->stream>>>(
-          s->reps_d, tbl, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot, ncounts, nlist, maxn);
-      break;
-  }
+  if (NEIGH || s->defer == 0)
+    launch_forces_k<NEIGH, 0>(s, ncounts, nlist, maxn, tbl, nblk);
+  else if (s->defer == 16)
+    launch_forces_k<false, 16>(s, ncounts, nlist, maxn, tbl, nblk);
+  else
+    launch_forces_k<false, 32>(s, ncounts, nlist, maxn, tbl, nblk);
 }
 
 template <int CAP>
@@ -1211,17 +1434,17 @@ void launch_brick_t(bd_sys* s) {
   switch (s->pot_kind) {
     case BD_POT_WCA:
       k_forces_brick<BD_POT_WCA, CAP><<<nblk, kBrickThreads, 0, s->stream>>>(
-          s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy,
+          s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy,
           s->fz, s->vir, s->pot, s->flags);
       break;
     case BD_POT_WCA_MODIFIED:
       k_forces_brick<BD_POT_WCA_MODIFIED, CAP><<<nblk, kBrickThreads, 0, s->stream>>>(
-          s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy,
+          s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy,
           s->fz, s->vir, s->pot, s->flags);
       break;
     default:
       k_forces_brick<BD_POT_HARMONIC, CAP><<<nblk, kBrickThreads, 0, s->stream>>>(
-          s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy,
+          s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy,
           s->fz, s->vir, s->pot, s->flags);
       break;
   }
@@ -1233,17 +1456,17 @@ void launch_split_t(bd_sys* s) {
   switch (s->pot_kind) {
     case BD_POT_WCA:
       k_forces_split<BD_POT_WCA, G><<<nblk, kBlk, 0, s->stream>>>(
-          s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy, s->fz,
+          s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz,
           s->vir, s->pot);
       break;
     case BD_POT_WCA_MODIFIED:
       k_forces_split<BD_POT_WCA_MODIFIED, G><<<nblk, kBlk, 0, s->stream>>>(
-          s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy, s->fz,
+          s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz,
           s->vir, s->pot);
       break;
     default:
       k_forces_split<BD_POT_HARMONIC, G><<<nblk, kBlk, 0, s->stream>>>(
-          s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->cnt, s->cpos, s->fx, s->fy, s->fz,
+          s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz,
           s->vir, s->pot);
       break;
   }
@@ -1350,7 +1573,7 @@ void free_all(bd_sys* s) {
   void* ptrs[] = {s->tbl_part, s->tbl_lane, s->tbl_cell, s->reps_d, s->px,   s->py,
                   s->pz,       s->rx,       s->ry,       s->rz,     s->ix,   s->iy,
                   s->iz,       s->fx,       s->fy,       s->fz,     s->vir,  s->pot,
-                  s->nocup,    s->cnt,      s->ins,      s->srt,    s->cpos, s->ring,
+                  s->nocup,    s->cnt,      s->ins,      s->rs,     s->cpos, s->ring,
                   s->lcg,      s->draws,    s->step,     s->flags};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1475,7 +1698,7 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
   s->have_pos.assign(n_replicas, false);
   s->pot_kind = params[0].potential;
   s->ni = params[0].first_step;
-  int64_t p_off = 0, lane_off = 0, cell_off = 0;
+  int64_t p_off = 0, lane_off = 0, cell_off = 0, rs_off = 0;
   int64_t tag_off = 0;
   for (int r = 0; r < n_replicas; r++) {
     const bd_params& p = params[r];
@@ -1509,8 +1732,10 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
       const int64_t nc3 = (int64_t)R.ncell[k] * R.ncell[k] * R.ncell[k];
       R.cell_off[k] = (int32_t)cell_off;
       R.tag_off[k] = (int32_t)tag_off;
+      R.rs_off[k] = (int32_t)rs_off;
       cell_off += nc3;
       tag_off += nc3 * R.ntags[k];
+      rs_off += (int64_t)R.ncell[k] * R.ncell[k] * (R.ncell[k] + 1);
     }
     R.nrange_same[0] = g.nrange_bb;
     R.nrange_same[1] = g.nrange_ss;
@@ -1548,18 +1773,25 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
     add_blocks(s->tbl_part_h, r, R.n, 0);
     add_blocks(s->tbl_lane_h, r, R.lanes, 0);
     for (int k = 0; k < 2; k++)
-      add_blocks(s->tbl_cell_h, r, R.ncell[k] * R.ncell[k] * R.ncell[k], k);
+      add_blocks(s->tbl_cell_h, r, R.ncell[k] * R.ncell[k], k, kRowsPerBlock);
   }
   s->tbl_part_h = xcd_group(s->tbl_part_h);
   s->tbl_lane_h = xcd_group(s->tbl_lane_h);
   s->tbl_cell_h = xcd_group(s->tbl_cell_h);
-  if (p_off > INT32_MAX / 2 || tag_off > INT32_MAX || lane_off > INT32_MAX / 2) {
+  if (p_off > INT32_MAX / 2 || tag_off > INT32_MAX || lane_off > INT32_MAX / 2 ||
+      rs_off > INT32_MAX || cell_off > INT32_MAX) {
     delete s;
     return fail(BD_E_ARG, "engine too large for 32-bit offsets");
   }
   s->n_total = (int)p_off;
   s->lanes_total = (int)lane_off;
   s->cells_total = (int)cell_off;
+  s->rs_total = (int)rs_off;
+  for (const RepDev& R : s->reps)
+    for (int k = 0; k < 2; k++)
+      if ((int64_t)R.ncell[k] * R.ncell[k] * R.ncell[k] * R.ntags[k] > (int64_t)kSlotMask)
+        s->defer_ok = false;
+  if (!s->defer_ok) s->defer = 0;
   s->tags_total = (size_t)tag_off;
   int rc = BD_OK;
 #define ALLOC(ptr, n)                 \
@@ -1575,7 +1807,8 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
   ALLOC(s->fx, N) ALLOC(s->fy, N) ALLOC(s->fz, N);
   ALLOC(s->vir, N) ALLOC(s->pot, N);
   ALLOC(s->nocup, (size_t)s->cells_total) ALLOC(s->cnt, (size_t)s->cells_total);
-  ALLOC(s->ins, s->tags_total) ALLOC(s->srt, s->tags_total) ALLOC(s->cpos, s->tags_total);
+  ALLOC(s->ins, s->tags_total) ALLOC(s->cpos, s->tags_total);
+  ALLOC(s->rs, (size_t)s->rs_total);
   ALLOC(s->ring, (size_t)bd::kRingLen * s->lanes_total) ALLOC(s->lcg, (size_t)s->lanes_total);
   ALLOC(s->draws, (size_t)n_replicas) ALLOC(s->step, 1) ALLOC(s->flags, (size_t)n_replicas);
 #undef ALLOC
@@ -1603,6 +1836,7 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
     if (e == hipSuccess) e = hipMemset(zt, 0, n8);
   if (e == hipSuccess) e = hipMemset(s->nocup, 0, sizeof(int) * s->cells_total);
   if (e == hipSuccess) e = hipMemset(s->cnt, 0, sizeof(int) * s->cells_total);
+  if (e == hipSuccess) e = hipMemset(s->rs, 0, sizeof(int) * s->rs_total);
   if (e == hipSuccess) e = hipMemset(s->draws, 0, sizeof(uint64_t) * n_replicas);
   if (e == hipSuccess) e = hipMemset(s->flags, 0, sizeof(int) * n_replicas);
   if (e == hipSuccess) e = hipMemcpy(s->step, &s->ni, sizeof(int64_t), hipMemcpyHostToDevice);
@@ -1775,17 +2009,26 @@ int bd_get_cells(bd_sys* s, int32_t replica, int32_t species, int32_t* nocup,
   const int n = R.ncell[species];
   const size_t nc3 = (size_t)n * n * n;
   const int ntags = R.ntags[species];
-  std::vector<int32_t> srt(nc3 * ntags);
+  const int sp_off = species == 0 ? 0 : R.nb;
+  std::vector<int32_t> rs((size_t)n * n * (n + 1));
+  std::vector<double4> cp(nc3 * ntags);
   HIP_TRY(hipMemcpyAsync(nocup, s->cnt + R.cell_off[species], sizeof(int32_t) * nc3,
                          hipMemcpyDeviceToHost, s->stream));
-  HIP_TRY(hipMemcpyAsync(srt.data(), s->srt + R.tag_off[species], sizeof(int32_t) * nc3 * ntags,
+  HIP_TRY(hipMemcpyAsync(rs.data(), s->rs + R.rs_off[species], sizeof(int32_t) * rs.size(),
+                         hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipMemcpyAsync(cp.data(), s->cpos + R.tag_off[species], sizeof(double4) * cp.size(),
                          hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
-  for (size_t c = 0; c < nc3; c++) {
-    const int occ = std::min(nocup[c], ntags);
-    for (int t = 0; t < ntags; t++)
-      cell_vec[c * ntags + t] = t < occ ? srt[c * ntags + t] : -1;
-  }
+  // packed rows back to the reference's ntags slots per cell
+  for (int z = 0; z < n; z++)
+    for (int y = 0; y < n; y++)
+      for (int x = 0; x < n; x++) {
+        const size_t c = (size_t)x + (size_t)n * (y + (size_t)n * z);
+        const size_t ri = ((size_t)x + (size_t)n * y) * (n + 1) + z;
+        const int s0 = rs[ri], occ = rs[ri + 1] - s0;
+        for (int t = 0; t < ntags; t++)
+          cell_vec[c * ntags + t] = t < occ ? (int32_t)cp[(size_t)s0 + t].w - sp_off : -1;
+      }
   return BD_OK;
 }
 
@@ -1980,13 +2223,14 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
   if (cache_path) {
     if (FILE* f = std::fopen(cache_path, "r")) {
       char k[200];
-      int b;
-      int found = -1;
-      while (std::fscanf(f, "%199s %d", k, &b) == 2)
-        if (key == k) found = b;
+      int b, d;
+      int found_b = -1, found_d = 0;
+      while (std::fscanf(f, "%199s %d %d", k, &b, &d) == 3)
+        if (key == k) found_b = b, found_d = d;
       std::fclose(f);
-      if (found >= 0 && bd_set_brick(s, found) == BD_OK) {
-        if (chosen) *chosen = found;
+      if (found_b >= 0 && bd_set_defer_depth(s, found_d) == BD_OK &&
+          bd_set_brick(s, found_b) == BD_OK) {
+        if (chosen) *chosen = found_b;
         return BD_OK;
       }
     }
@@ -1998,16 +2242,28 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
     launch_cells(s, 0);
     s->located = true;
   }
-  int best = 0;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->graph) {
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_steps = 0;
+  }
+  // kernel shapes: the row walk at three deferral depths, the LDS bricks
+  struct Shape {
+    int brick, defer;
+  };
+  const Shape candidates[] = {{0, 0}, {0, 16}, {0, 32}, {1, 0}, {2, 0}, {3, 0}, {4, 0}};
+  Shape best{0, 0};
   double best_ms = 1e300;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  const int candidates[] = {0, 1, 2, 3, 4};
-  const int keep = s->brick;
-  for (int b : candidates) {
-    if (s->order != BD_ORDER_REFERENCE && b != 0) continue;
-    if (setup_bricks(s, b) != BD_OK) continue;
+  const Shape keep{s->brick, s->defer};
+  for (const Shape& c : candidates) {
+    if (s->order != BD_ORDER_REFERENCE && (c.brick != 0 || c.defer != keep.defer)) continue;
+    if (c.defer != 0 && !s->defer_ok) continue;
+    if (setup_bricks(s, c.brick) != BD_OK) continue;
+    s->defer = c.defer;
     launch_forces(s);  // warm-up
     HIP_TRY(hipEventRecord(e0, s->stream));
     for (int k = 0; k < trial_launches; k++) launch_forces(s);
@@ -2017,23 +2273,46 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     if (ms < best_ms) {
       best_ms = ms;
-      best = b;
+      best = c;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   g_err.clear();
-  rc = bd_set_brick(s, best);
+  s->defer = best.defer;
+  rc = bd_set_brick(s, best.brick);
   if (rc != BD_OK) {
-    bd_set_brick(s, keep);
+    s->defer = keep.defer;
+    bd_set_brick(s, keep.brick);
     return rc;
   }
   if (cache_path)
     if (FILE* f = std::fopen(cache_path, "a")) {
-      std::fprintf(f, "%s %d\n", key.c_str(), best);
+      std::fprintf(f, "%s %d %d\n", key.c_str(), best.brick, best.defer);
       std::fclose(f);
     }
-  if (chosen) *chosen = best;
+  if (chosen) *chosen = best.brick;
+  return BD_OK;
+}
+
+int bd_set_defer_depth(bd_sys* s, int32_t depth) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (depth != 0 && depth != 16 && depth != 32) return fail(BD_E_ARG, "depth must be 0, 16 or 32");
+  if (depth != 0 && !s->defer_ok)
+    return fail(BD_E_ARG, "a grid has too many slots for deferred pair records");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->graph) {
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_steps = 0;
+  }
+  s->defer = depth;
+  return BD_OK;
+}
+
+int bd_get_defer_depth(const bd_sys* s, int32_t* depth) {
+  if (!s || !depth) return fail(BD_E_ARG, "null argument");
+  *depth = s->defer;
   return BD_OK;
 }
 

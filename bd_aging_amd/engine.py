@@ -197,6 +197,15 @@ class Engine:
         check(self.lib.bd_get_graph_steps(self.h, C.byref(v)))
         return v.value
 
+    def set_defer_depth(self, depth):
+        """Deferred pair terms of the row walk: 0, 16 or 32 LDS entries."""
+        check(self.lib.bd_set_defer_depth(self.h, int(depth)))
+
+    def defer_depth(self):
+        v = C.c_int32()
+        check(self.lib.bd_get_defer_depth(self.h, C.byref(v)))
+        return v.value
+
     def set_brick(self, brick):
         check(self.lib.bd_set_brick(self.h, int(brick)))
 

@@ -56,6 +56,9 @@ def run_plain(steps, warmup, graph_steps=16):
     eng.set_positions(x)
     log(f"[bench] warmup {warmup} steps at T=1.0")
     eng.step(warmup)
+    brick = eng.autotune(20)
+    shape = {"brick": brick, "defer_depth": eng.defer_depth()}
+    log(f"[bench] autotuned force kernel {shape}")
     eng.mark_origin()
     log(f"[bench] timed {steps} steps at T=0.1")
     t0 = time.perf_counter()
@@ -66,7 +69,7 @@ def run_plain(steps, warmup, graph_steps=16):
     obs = eng.observables(7.0)
     eng.close()
     return {"N": nb + ns, "side": side, "seconds": dt, "rate": (nb + ns) * steps / dt,
-            "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p}
+            "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p, "shape": shape}
 
 
 def run_batch(steps, warmup, graph_steps=16):
@@ -86,6 +89,9 @@ def run_batch(steps, warmup, graph_steps=16):
     for r, x in enumerate(xs):
         eng.set_positions(x, r)
     eng.step(warmup)
+    brick = eng.autotune(10)
+    shape = {"brick": brick, "defer_depth": eng.defer_depth()}
+    log(f"[bench] batch autotuned force kernel {shape}")
     eng.mark_origin()
     log(f"[bench] batch timed {steps} steps")
     t0 = time.perf_counter()
@@ -96,7 +102,7 @@ def run_batch(steps, warmup, graph_steps=16):
     eng.close()
     tot = int(n.sum())
     return {"replicas": len(n), "sum_N": tot, "seconds": dt, "rate": tot * steps / dt,
-            "kernel_ms": ms.tolist()}
+            "kernel_ms": ms.tolist(), "shape": shape}
 
 
 def cpu_baseline(plain, budget_s=10.0):
@@ -163,7 +169,8 @@ def main():
         "data": "synthetic: seeded random placement + overlap removal, then BD",
         "config": {"workload": "aging quench N=32768 phi=0.58 WCA sigma 1.4/1.0, dt=1e-4, "
                                "T 1.0 -> 0.1", "N": N, "side": round(plain["side"], 4),
-                   "potential": "wca", "cells": "23^3 big + 32^3 small, ntags 8"},
+                   "potential": "wca", "cells": "23^3 big + 32^3 small, ntags 8",
+                   "force_kernel": plain["shape"]},
         "roofline": {"bound": "hbm", "kernel": "k_forces", "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": None, "bytes_per_particle": B_FORCES,
@@ -178,7 +185,8 @@ def main():
                         "replicas": batch["replicas"], "sum_N": batch["sum_N"],
                         "steps": a.batch_steps, "ratio_vs_plain": batch["rate"] / plain["rate"],
                         "step_hbm_frac": B_STEP * batch["rate"] / HBM_PEAK,
-                        "kernel_ms": dict(zip(["update", "cells", "forces"], batch["kernel_ms"]))}
+                        "kernel_ms": dict(zip(["update", "cells", "forces"], batch["kernel_ms"])),
+                        "force_kernel": batch["shape"]}
     print(json.dumps(out), flush=True)
     if a.copy_calibration:
         copy_calibration()

@@ -143,18 +143,28 @@ int bd_step(bd_sys* s, int32_t nsteps);
 int bd_set_graph_steps(bd_sys* s, int32_t steps);
 int bd_get_graph_steps(const bd_sys* s, int32_t* steps);
 
-/* Force-kernel shape (no reference counterpart; the autotuner's knob):
+/* Force-kernel shape (no reference counterpart; the autotuner's knobs):
  * brick = 1..4 selects the LDS-staged kernel over brick^3 big-grid cells,
- * 0 the thread-per-particle kernel.  Every choice gives bit-identical
+ * 0 the thread-per-particle row walk.  Every choice gives bit-identical
  * results. */
 int bd_set_brick(bd_sys* s, int32_t brick);
 int bd_get_brick(const bd_sys* s, int32_t* brick);
 
+/* Deferral depth of the row walk (no reference counterpart): 0 evaluates
+ * each pair term where it is found; 16 or 32 (default 16) records the
+ * partners inside the cutoff in LDS and evaluates them in the same order
+ * afterwards.  Bit-identical results at every depth.  BD_E_ARG for other
+ * values, or for a nonzero depth when a grid holds 2^26 slots or more. */
+int bd_set_defer_depth(bd_sys* s, int32_t depth);
+int bd_get_defer_depth(const bd_sys* s, int32_t* depth);
+
 /* Autotuner (no reference counterpart): measures the force kernel for every
- * brick shape (bit-identical results by construction) with forces-only
- * launches on the current state, keeps the fastest and returns it in
- * *chosen.  With cache_path != NULL the choice is looked up in / appended to
- * that text file, keyed by (replicas, N, phi, potential, order). */
+ * shape (row walk at deferral depths 0, 16 and 32; bricks 1..4; all
+ * bit-identical by construction) with forces-only launches on the current
+ * state, keeps the fastest and returns its brick in *chosen (the depth is
+ * read with bd_get_defer_depth).  With cache_path != NULL the choice is
+ * looked up in / appended to that text file as "key brick depth", keyed by
+ * (replicas, N, phi, potential, order). */
 int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32_t* chosen);
 
 /* Force summation order (no reference counterpart).

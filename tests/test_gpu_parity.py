@@ -136,11 +136,53 @@ def test_every_brick_shape_bit_identical(config1_positions, pot):
             continue
         tried.append(b)
         assert e.brick() == b
+        for depth in ((0, 16, 32) if b == 0 else (16,)):
+            e.set_defer_depth(depth)
+            assert e.defer_depth() == depth
+            e.compute_forces()
+            fe, ve, ue = e.forces()
+            assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and \
+                np.array_equal(ue, uo), (b, depth)
+    assert {0, 1, 2, 3} <= set(tried)
+    for bad in (-1, 8, 64):
+        with pytest.raises(BDError):
+            e.set_defer_depth(bad)
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def plain_positions():
+    """N=32768 (16384 + 16384) at phi=0.58, no overlaps: the bench workload."""
+    side = box_side(16384, 16384, 1.4, 1.0, 0.58)
+    return side, relax_overlaps(16384, 16384, side, seed=1)
+
+
+def test_forces_bit_exact_at_bench_size(plain_positions):
+    """Every force-kernel shape on the 23^3 + 32^3-cell grids, against the oracle."""
+    side, x = plain_positions
+    p = make_params(16384, 16384, side=side, seed=6)
+    o = orc.System(p)
+    o.set_positions(x)
+    o.cell_locate()
+    o.compute_forces()
+    fo, vo, uo = o.forces()
+    e = Engine(p)
+    e.set_positions(x)
+    e.cell_locate()
+    shapes = [(0, 0), (0, 16), (0, 32), (1, 16), (2, 16)]
+    for b, depth in shapes:
+        e.set_brick(b)
+        e.set_defer_depth(depth)
         e.compute_forces()
         fe, ve, ue = e.forces()
-        assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and np.array_equal(ue, uo), b
-    assert {0, 1, 2, 3} <= set(tried)
+        assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and np.array_equal(ue, uo), \
+            (b, depth)
+    for sp in (0, 1):
+        ne, ce = e.cells(sp)
+        no, co = o.cells(sp)
+        assert np.array_equal(ne, no) and np.array_equal(ce, co)
     e.close()
+    o.close()
 
 
 # ------------------------------------------------------------------ update
@@ -383,7 +425,8 @@ def test_autotune_keeps_results_and_caches(config1_positions, tmp_path):
     e.compute_forces()
     f1 = e.forces()
     assert all(np.array_equal(a, c) for a, c in zip(f0, f1))
-    assert open(cache).read().split()[-1] == str(b)
+    last = open(cache).read().split()
+    assert last[-2] == str(b) and last[-1] == str(e.defer_depth())
     e2 = Engine(p)
     e2.set_positions(x)
     assert e2.autotune(5, cache) == b  # served from the cache
