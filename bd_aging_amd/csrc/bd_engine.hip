@@ -262,15 +262,18 @@ __global__ void __launch_bounds__(kBlk)
   if (bad) atomicOr(&flags[t.x], kFlagNonFinite);
 }
 
-// Row packing, one wavefront per row (x, y) of a grid: lane z reads the
-// occupancy of cell (x, y, z), a wavefront prefix sum places the cells one
-// after the other from the row's first slot, and each lane writes its
-// cell's particles ranked by id (ascending, the serial order of
-// cell_locate_hst, cell_locate.cu:72-101) with their positions.  Rows longer
-// than 64 cells are packed in chunks of 64.  The insertion counters are
+// Row packing.  A wavefront packs rpw = 64 / n rows (x, y) of a grid with n
+// <= 64 cells per row (lane j n + z holds cell z of its j-th row), or one
+// row in chunks of 64 cells when n > 64.  Each lane reads its cell's
+// occupancy, a segmented wavefront prefix sum places the cells of a row one
+// after the other from the row's first slot, and the lane writes its cell's
+// particles ranked by id (ascending, the serial order of cell_locate_hst,
+// cell_locate.cu:72-101) with their positions.  The insertion counters are
 // reset for the next binning.  With advance != 0 the first thread also
 // advances the step index and the per-replica draw counters.
-constexpr int kRowsPerBlock = kBlk / 64;
+constexpr int kRowWaves = kBlk / 64;
+
+__host__ __device__ inline int rows_per_wave(int n) { return n <= 64 ? 64 / n : 1; }
 
 __global__ void __launch_bounds__(kBlk)
     k_cells(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
@@ -285,12 +288,19 @@ __global__ void __launch_bounds__(kBlk)
   }
   const int4 t = tbl[blockIdx.x];
   if (t.x < 0) return;
-  const int row = t.y + (int)threadIdx.x / 64;  // x + n y
-  if (row >= t.z) return;                         // whole wavefront
-  const int lane = (int)threadIdx.x % 64;
   const int g = t.w;
   const RepDev& R = reps[t.x];
   const int n = R.ncell[g];
+  const int rpw = rows_per_wave(n);
+  const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+  const int row0 = t.y + wave * rpw;
+  if (row0 >= t.z) return;  // whole wavefront
+  const int j = n <= 64 ? lane / n : 0;
+  const int seg0 = j * n;  // first lane of this lane's row (n <= 64), else 0
+  const int zl = lane - seg0;
+  const int row = row0 + j;  // x + n y
+  const bool row_live = j < rpw && row < t.z;
+  const int seg_last = min(seg0 + min(n, 64) - 1, 63);
   const int ntags = R.ntags[g];
   const int sp_off = g == 0 ? 0 : R.nb;
   const int row_base = row * n * ntags;  // grid-relative slot of the row
@@ -300,8 +310,8 @@ __global__ void __launch_bounds__(kBlk)
   int carry = 0;
   bool over = false;
   for (int z0 = 0; z0 < n; z0 += 64) {
-    const int z = z0 + lane;
-    const bool live = z < n;
+    const int z = z0 + zl;
+    const bool live = row_live && z < n;
     const int c = row + n * n * z;  // x + n (y + n z)
     int occ = 0;
     if (live) occ = nocup[R.cell_off[g] + c];
@@ -312,14 +322,17 @@ __global__ void __launch_bounds__(kBlk)
       const int v = __shfl_up(incl, off, 64);
       if (lane >= off) incl += v;
     }
-    const int start = row_base + carry + incl - m;
+    const int before = __shfl(incl, seg0 > 0 ? seg0 - 1 : 0, 64);
+    const int prior = seg0 > 0 ? before : 0;
+    const int row_sum = __shfl(incl, seg_last, 64) - prior;
+    const int start = row_base + carry + (incl - prior) - m;
     if (live) {
       rrow[z] = start;
       const int* slots = ins_g + (size_t)c * ntags;
       for (int k = 0; k < m; k++) {
         const int idk = slots[k];
         int rank = 0;
-        for (int j = 0; j < m; j++) rank += slots[j] < idk ? 1 : 0;
+        for (int q = 0; q < m; q++) rank += slots[q] < idk ? 1 : 0;
         const int p = R.p_off + sp_off + idk;
         cp[start + rank] = make_double4(px[p], py[p], pz[p], (double)(sp_off + idk));
       }
@@ -327,9 +340,9 @@ __global__ void __launch_bounds__(kBlk)
       nocup[R.cell_off[g] + c] = 0;
       over = over || occ >= ntags;
     }
-    carry += __shfl(incl, 63, 64);
+    carry += row_sum;
   }
-  if (lane == 0) rrow[n] = row_base + carry;
+  if (row_live && zl == 0) rrow[n] = row_base + carry;
   if (over) atomicOr(&flags[t.x], kFlagOverflow);
 }
 
@@ -697,10 +710,14 @@ __device__ inline void scan_rows(const RepDev& R, int n, double cs, double sigma
       const bool center = self >= 0 && di == 0 && dj == 0;
       const unsigned int cxy =
           (image_code(shx) << kSlotBits) | (image_code(shy) << (kSlotBits + 2));
+      // the next candidate is loaded while this one is evaluated
+      double4 qn = make_double4(0.0, 0.0, 0.0, -1.0);
+      if (tot > 0) qn = cp[na > 0 ? s0a : s0b];
       for (int t = 0; t < tot; t++) {
         const bool in_a = t < na;
         const int sl = in_a ? s0a + t : s0b + (t - na);
-        const double4 q = cp[sl];
+        const double4 q = qn;
+        if (t + 1 < tot) qn = cp[t + 1 < na ? s0a + t + 1 : s0b + (t + 1 - na)];
         if (center && q.w == (double)self) continue;
         const double shz = in_a ? sha : shb;
         double dx = q.x - x;
@@ -1773,7 +1790,8 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
     add_blocks(s->tbl_part_h, r, R.n, 0);
     add_blocks(s->tbl_lane_h, r, R.lanes, 0);
     for (int k = 0; k < 2; k++)
-      add_blocks(s->tbl_cell_h, r, R.ncell[k] * R.ncell[k], k, kRowsPerBlock);
+      add_blocks(s->tbl_cell_h, r, R.ncell[k] * R.ncell[k], k,
+                 kRowWaves * rows_per_wave(R.ncell[k]));
   }
   s->tbl_part_h = xcd_group(s->tbl_part_h);
   s->tbl_lane_h = xcd_group(s->tbl_lane_h);

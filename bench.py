@@ -28,8 +28,12 @@ from bd_aging_amd.engine import Engine, make_params, box_side  # noqa: E402
 from bd_aging_amd.init import relax_overlaps, relax_batch  # noqa: E402
 
 HBM_PEAK = 8.0e12          # B/s, MI355X
+FP64_PEAK = 78.6e12        # FLOP/s, MI355X vector FP64
 B_FORCES = 72              # algorithmic bytes per particle, force pass
+B_BIN = 36                 # algorithmic bytes per particle, binning pass
+B_UPDATE = 120             # algorithmic bytes per particle, update pass (ring excluded)
 B_STEP = 228               # algorithmic bytes per particle-step (bin 36 + forces 72 + update 120)
+FLOP_PAIR = 22             # FP64 operations per ordered pair in the g(r) scan
 
 
 def log(msg):
@@ -67,9 +71,16 @@ def run_plain(steps, warmup, graph_steps=16):
     ksteps = min(200, steps)
     ms = eng.kernel_times(ksteps) / ksteps
     obs = eng.observables(7.0)
+    eng.gder_counts(128, 5.0)  # warm-up
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        eng.gder_counts(128, 5.0)
+    gder_s = (time.perf_counter() - t0) / reps
     eng.close()
     return {"N": nb + ns, "side": side, "seconds": dt, "rate": (nb + ns) * steps / dt,
-            "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p, "shape": shape}
+            "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p, "shape": shape,
+            "gder_s": gder_s}
 
 
 def run_batch(steps, warmup, graph_steps=16):
@@ -105,6 +116,36 @@ def run_batch(steps, warmup, graph_steps=16):
             "kernel_ms": ms.tolist(), "shape": shape}
 
 
+def run_stress(steps, warmup, graph_steps=16):
+    """N=262144 at phi=0.58, T=0.5, dt=5e-5: per-kernel times and HBM fractions."""
+    nb = ns = 131072
+    side = box_side(nb, ns, 1.4, 1.0, 0.58)
+    log(f"[bench] stress: relaxing N=262144 phi=0.58 L={side:.4f}")
+    x = relax_overlaps(nb, ns, side, seed=3, graph_steps=graph_steps)
+    p = make_params(nb, ns, side=side, dt=5e-5, T0=0.5, Tf=0.5, first_step=0, seed=4242)
+    eng = Engine(p)
+    eng.set_graph_steps(graph_steps)
+    eng.set_positions(x)
+    eng.step(warmup)
+    brick = eng.autotune(10)
+    shape = {"brick": brick, "defer_depth": eng.defer_depth()}
+    t0 = time.perf_counter()
+    eng.step(steps)
+    dt = time.perf_counter() - t0
+    ms = eng.kernel_times(100) / 100
+    geo = eng.geometry()
+    eng.close()
+    N = nb + ns
+    kern = {}
+    for name, b, m in (("update", B_UPDATE, ms[0]), ("binning", B_BIN, ms[1]),
+                       ("forces", B_FORCES, ms[2])):
+        gbs = b * N / (m * 1e-3)
+        kern[name] = {"ms": m, "bytes_per_particle": b, "GB_s": gbs / 1e9, "hbm_frac": gbs / HBM_PEAK}
+    return {"N": N, "side": round(side, 4), "steps": steps, "rate": N * steps / dt,
+            "cells": f"{geo.ncell_big}^3 big + {geo.ncell_sml}^3 small",
+            "force_kernel": shape, "kernels": kern}
+
+
 def cpu_baseline(plain, budget_s=10.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
@@ -133,6 +174,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stress-steps", type=int, default=1000,
+                    help="steps of the N=262144 stress configuration (0 skips it)")
     ap.add_argument("--graph-steps", type=int, default=16,
                     help="steps per captured HIP graph; 0 launches every kernel eagerly "
                          "(same kernels, for kernel tracing)")
@@ -150,6 +193,10 @@ def main():
     if not a.no_batch:
         batch = run_batch(a.batch_steps, min(a.warmup, 500), a.graph_steps)
         log(f"[bench] batch: {batch['rate']:.4e} particle-steps/s")
+    stress = None
+    if a.stress_steps > 0:
+        stress = run_stress(a.stress_steps, 100, a.graph_steps)
+        log(f"[bench] stress: {stress['rate']:.4e} particle-steps/s")
     cpu = None if a.no_cpu else cpu_baseline(plain)
     N = plain["N"]
     f_ms = plain["kernel_ms"][2]
@@ -178,8 +225,14 @@ def main():
         "kernel_ms": {"update": plain["kernel_ms"][0], "cells": plain["kernel_ms"][1],
                       "forces": plain["kernel_ms"][2]},
         "step_hbm_frac": B_STEP * plain["rate"] / HBM_PEAK,
+        "gder": {"pairs_per_s": N * (N - 1) / plain["gder_s"],
+                 "GFLOP_s": FLOP_PAIR * N * (N - 1) / plain["gder_s"] / 1e9,
+                 "fp64_frac": FLOP_PAIR * N * (N - 1) / plain["gder_s"] / FP64_PEAK,
+                 "flop_per_pair": FLOP_PAIR, "bins": 128, "range": 5.0},
         "cpu_baseline": cpu,
     }
+    if stress:
+        out["stress"] = stress
     if batch:
         out["batch"] = {"value": batch["rate"], "unit": "particle-steps/s",
                         "replicas": batch["replicas"], "sum_N": batch["sum_N"],
