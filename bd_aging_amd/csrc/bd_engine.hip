@@ -53,7 +53,10 @@
 
 namespace {
 
-constexpr int kBlk = 256;
+// One wavefront per workgroup for the per-particle, per-lane and per-row
+// kernels: a 32768-particle system then spreads over 512 workgroups, i.e.
+// every CU of the chip, instead of 128.
+constexpr int kBlk = 64;
 constexpr int kFlagOverflow = 1;
 constexpr int kFlagNonFinite = 2;
 constexpr int kFlagBrick = 4;
@@ -625,12 +628,9 @@ __device__ inline void drain(const RepDev& R, double sigma, double x, double y, 
     const double4 q = cp[w & kSlotMask];
     const unsigned int cx = (w >> kSlotBits) & 3u, cy = (w >> (kSlotBits + 2)) & 3u,
                        cz = w >> (kSlotBits + 4);
-    double dx = q.x - x;
-    double dy = q.y - y;
-    double dz = q.z - z;
-    if (cx != 0u) dx = dx + (cx == 1u ? -L : L);
-    if (cy != 0u) dy = dy + (cy == 1u ? -L : L);
-    if (cz != 0u) dz = dz + (cz == 1u ? -L : L);
+    const double dx = (q.x - x) + (cx == 0u ? 0.0 : (cx == 1u ? -L : L));
+    const double dy = (q.y - y) + (cy == 0u ? 0.0 : (cy == 1u ? -L : L));
+    const double dz = (q.z - z) + (cz == 0u ? 0.0 : (cz == 1u ? -L : L));
     const double dist2 = dx * dx + dy * dy + dz * dz;
     pair_add<POT>(R, sigma, dist2, dx, dy, dz, fx, fy, fz, vir, pot);
   }
@@ -720,12 +720,11 @@ __device__ inline void scan_rows(const RepDev& R, int n, double cs, double sigma
         if (t + 1 < tot) qn = cp[t + 1 < na ? s0a + t + 1 : s0b + (t + 1 - na)];
         if (center && q.w == (double)self) continue;
         const double shz = in_a ? sha : shb;
-        double dx = q.x - x;
-        double dy = q.y - y;
-        double dz = q.z - z;
-        if (shx != 0.0) dx = dx + shx;
-        if (shy != 0.0) dy = dy + shy;
-        if (shz != 0.0) dz = dz + shz;
+        // adding a zero shift changes at most the sign of a zero difference,
+        // which no squared distance, force or virial term can see
+        const double dx = (q.x - x) + shx;
+        const double dy = (q.y - y) + shy;
+        const double dz = (q.z - z) + shz;
         const double dist2 = dx * dx + dy * dy + dz * dz;
         if (dist2 < rc2) {
           if (DEPTH == 0) {

@@ -203,12 +203,19 @@ class Surrogate:
         self.mu = None
         self.sd = None
 
-    def fit(self, X, Y, epochs=60, batch=256, lr=1e-3, seed=1):
+    def fit(self, X, Y, epochs=150, batch=256, lr=1e-3, seed=1):
         """Adam in four phases of epochs/4 with the rate stepped down
         (lr, 0.3 lr, 0.1 lr, 0.03 lr); returns the per-epoch losses."""
         self.mu = X.mean(0)
-        self.sd = X.std(0) + 1e-6
-        Xn = (X - self.mu) / self.sd
+        # The g(r) block is scaled as a whole (one scale for all 3 x 128 bins),
+        # so a bin that barely varies (below contact, or noise at small N)
+        # stays a small input instead of being blown up to unit variance; T,
+        # phi and log10(t_w) are standardised one by one.
+        ng = 3 * N_BINS
+        self.sd = np.empty(X.shape[1])
+        self.sd[:ng] = max(float((X[:, :ng] - self.mu[:ng]).std()), 1e-6)
+        self.sd[ng:] = np.maximum(X[:, ng:].std(0), 1e-6)
+        Xn = self.standardise(X)
         phases = [max(1, epochs // 4)] * 3
         phases.append(max(1, epochs - sum(phases)))
         out = []
@@ -216,9 +223,12 @@ class Surrogate:
             out.append(self.net.train(Xn, Y, ep, batch, lr * f, seed + 7919 * k))
         return np.concatenate(out)
 
+    def standardise(self, X):
+        return np.clip((X - self.mu) / self.sd, -30.0, 30.0)
+
     def predict(self, X):
         """Ensemble mean, std and the fallback mask (True: run plain BD)."""
-        mean, std = self.net.predict((X - self.mu) / self.sd)
+        mean, std = self.net.predict(self.standardise(X))
         return mean, std, std.max(1) > self.std_gate
 
 
@@ -255,7 +265,7 @@ def main(argv=None):
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--nmin", type=int, default=1000)
     ap.add_argument("--nmax", type=int, default=16000)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--out", default="surrogate_out")
     ap.add_argument("--save-data", action="store_true",
                     help="also write the generated data set (dataset.npz) to --out")

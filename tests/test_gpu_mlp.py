@@ -76,3 +76,31 @@ def test_dataset_generation_small():
     assert np.all(g[:, :, r < 0.7] == 0)          # no overlaps
     assert g[:, 2, :].max() > 1.5                 # a first peak
     assert cost > 0
+
+
+def test_surrogate_gain_over_simulation():
+    """What the trained network gains over simulating: trained only on batched-BD
+    data, it predicts held-out F_s(k=7, t_w, t) curves of unseen state points
+    (N, phi, T, seed) much closer than the training curves' mean at the same t_w
+    (the best guess without the network), at >= 100x lower cost than the BD
+    that measured them; points the ensemble disagrees on go back to BD."""
+    from bd_aging_amd import surrogate as sg
+    runs = [sg.generate(R=64, n_range=(3000, 6000), seed=500 + k) for k in range(6)]
+    Xtr = np.concatenate([r[0] for r in runs[:5]])
+    Ytr = np.concatenate([r[1] for r in runs[:5]])
+    Xte, Yte, _, wall = runs[5]
+    sur = sg.Surrogate(seed=3)
+    sur.fit(Xtr, Ytr, epochs=300)
+    rep = sg.evaluate(sur, Xte, Yte, wall / len(Xte))
+    # without the network: the mean training curve at the same waiting time
+    tw = Xtr[:, -1]
+    naive = np.stack([Ytr[tw == v].mean(0) for v in Xte[:, -1]])
+    rms_naive = float(np.sqrt(np.mean((naive - Yte) ** 2)))
+    print(f"[surrogate gain] rms {rep['rms_all']:.4f} vs naive {rms_naive:.4f}, "
+          f"fallback {rep['fallback_fraction']:.3f}, cost ratio "
+          f"{rep['cost_ratio_bd_over_surrogate']:.0f}x", flush=True)
+    assert rep["rms_all"] < 0.6 * rms_naive
+    assert rep["rms_all"] < 0.04
+    assert rep["cost_ratio_bd_over_surrogate"] >= 100
+    assert rep["rms_with_fallback"] <= rep["rms_all"] + 1e-12
+    sur.net.close()
