@@ -708,29 +708,42 @@ __device__ inline void scan_rows(const RepDev& R, int n, double cs, double sigma
       const int na = s1a - s0a;
       const int tot = na + (s1b - s0b);
       const bool center = self >= 0 && di == 0 && dj == 0;
-      const unsigned int cxy =
-          (image_code(shx) << kSlotBits) | (image_code(shy) << (kSlotBits + 2));
       // the next candidate is loaded while this one is evaluated
       double4 qn = make_double4(0.0, 0.0, 0.0, -1.0);
       if (tot > 0) qn = cp[na > 0 ? s0a : s0b];
-      for (int t = 0; t < tot; t++) {
-        const bool in_a = t < na;
-        const int sl = in_a ? s0a + t : s0b + (t - na);
-        const double4 q = qn;
-        if (t + 1 < tot) qn = cp[t + 1 < na ? s0a + t + 1 : s0b + (t + 1 - na)];
-        if (center && q.w == (double)self) continue;
-        const double shz = in_a ? sha : shb;
-        // adding a zero shift changes at most the sign of a zero difference,
-        // which no squared distance, force or virial term can see
-        const double dx = (q.x - x) + shx;
-        const double dy = (q.y - y) + shy;
-        const double dz = (q.z - z) + shz;
-        const double dist2 = dx * dx + dy * dy + dz * dz;
-        if (dist2 < rc2) {
-          if (DEPTH == 0) {
-            pair_add<POT>(R, sigma, dist2, dx, dy, dz, fx, fy, fz, vir, pot);
-          } else {
-            buf[nd * kBlk] = (unsigned int)sl | cxy | (image_code(shz) << (kSlotBits + 4));
+      if (DEPTH == 0) {
+        for (int t = 0; t < tot; t++) {
+          const bool in_a = t < na;
+          const double4 q = qn;
+          if (t + 1 < tot) qn = cp[t + 1 < na ? s0a + t + 1 : s0b + (t + 1 - na)];
+          if (center && q.w == (double)self) continue;
+          const double shz = in_a ? sha : shb;
+          // adding a zero shift changes at most the sign of a zero difference,
+          // which no squared distance, force or virial term can see
+          const double dx = (q.x - x) + shx;
+          const double dy = (q.y - y) + shy;
+          const double dz = (q.z - z) + shz;
+          const double dist2 = dx * dx + dy * dy + dz * dz;
+          if (dist2 < rc2) pair_add<POT>(R, sigma, dist2, dx, dy, dz, fx, fy, fz, vir, pot);
+        }
+      } else {
+        // partners inside the cutoff are recorded in visiting order
+        const unsigned int cxy =
+            (image_code(shx) << kSlotBits) | (image_code(shy) << (kSlotBits + 2));
+        const unsigned int ca = cxy | (image_code(sha) << (kSlotBits + 4));
+        const unsigned int cb = cxy | (image_code(shb) << (kSlotBits + 4));
+        const double selfw = center ? (double)self : -1.0;
+        for (int t = 0; t < tot; t++) {
+          const bool in_a = t < na;
+          const int sl = in_a ? s0a + t : s0b + (t - na);
+          const double4 q = qn;
+          if (t + 1 < tot) qn = cp[t + 1 < na ? s0a + t + 1 : s0b + (t + 1 - na)];
+          const double dx = (q.x - x) + shx;
+          const double dy = (q.y - y) + shy;
+          const double dz = (q.z - z) + (in_a ? sha : shb);
+          const double dist2 = dx * dx + dy * dy + dz * dz;
+          if (dist2 < rc2 && q.w != selfw) {
+            buf[nd * kBlk] = (unsigned int)sl | (in_a ? ca : cb);
             if (++nd == DEPTH) {
               drain<POT>(R, sigma, x, y, z, cp, buf, nd, fx, fy, fz, vir, pot);
               nd = 0;
