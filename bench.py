@@ -34,6 +34,10 @@ B_BIN = 36                 # algorithmic bytes per particle, binning pass
 B_UPDATE = 120             # algorithmic bytes per particle, update pass (ring excluded)
 B_STEP = 228               # algorithmic bytes per particle-step (bin 36 + forces 72 + update 120)
 FLOP_PAIR = 22             # FP64 operations per ordered pair in the g(r) scan
+# HBM bytes per force launch (row walk, deferral depth 16) from the rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs in profiles/k_forces_traffic_pmc.txt, scaled by a copy kernel of known size
+TRAFFIC_FORCES_PLAIN = 6.46e6
+TRAFFIC_FORCES_BATCH = 1.491e8
 
 
 def log(msg):
@@ -48,7 +52,19 @@ def batch_specs(R=256, seed=20251203):
     return n, phi, T
 
 
-def run_plain(steps, warmup, graph_steps=16):
+def pick_shape(eng, shape, trials):
+    """Autotune the force kernel, or set the given (brick, deferral depth)."""
+    if shape is None:
+        brick = eng.autotune(trials)
+    else:
+        eng.set_brick(shape[0])
+        eng.set_defer_depth(shape[1])
+        brick = shape[0]
+    return {"brick": brick, "defer_depth": eng.defer_depth(),
+            "chosen_by": "autotuner" if shape is None else "--shape"}
+
+
+def run_plain(steps, warmup, graph_steps=16, shape=None):
     nb = ns = 16384
     side = box_side(nb, ns, 1.4, 1.0, 0.58)
     log(f"[bench] relaxing N=32768 phi=0.58 L={side:.4f}")
@@ -60,9 +76,8 @@ def run_plain(steps, warmup, graph_steps=16):
     eng.set_positions(x)
     log(f"[bench] warmup {warmup} steps at T=1.0")
     eng.step(warmup)
-    brick = eng.autotune(20)
-    shape = {"brick": brick, "defer_depth": eng.defer_depth()}
-    log(f"[bench] autotuned force kernel {shape}")
+    shape = pick_shape(eng, shape, 20)
+    log(f"[bench] force kernel {shape}")
     eng.mark_origin()
     log(f"[bench] timed {steps} steps at T=0.1")
     t0 = time.perf_counter()
@@ -83,7 +98,7 @@ def run_plain(steps, warmup, graph_steps=16):
             "gder_s": gder_s}
 
 
-def run_batch(steps, warmup, graph_steps=16):
+def run_batch(steps, warmup, graph_steps=16, shape=None):
     n, phi, T = batch_specs()
     specs, params = [], []
     for r in range(len(n)):
@@ -100,9 +115,8 @@ def run_batch(steps, warmup, graph_steps=16):
     for r, x in enumerate(xs):
         eng.set_positions(x, r)
     eng.step(warmup)
-    brick = eng.autotune(10)
-    shape = {"brick": brick, "defer_depth": eng.defer_depth()}
-    log(f"[bench] batch autotuned force kernel {shape}")
+    shape = pick_shape(eng, shape, 10)
+    log(f"[bench] batch force kernel {shape}")
     eng.mark_origin()
     log(f"[bench] batch timed {steps} steps")
     t0 = time.perf_counter()
@@ -116,7 +130,7 @@ def run_batch(steps, warmup, graph_steps=16):
             "kernel_ms": ms.tolist(), "shape": shape}
 
 
-def run_stress(steps, warmup, graph_steps=16):
+def run_stress(steps, warmup, graph_steps=16, shape=None):
     """N=262144 at phi=0.58, T=0.5, dt=5e-5: per-kernel times and HBM fractions."""
     nb = ns = 131072
     side = box_side(nb, ns, 1.4, 1.0, 0.58)
@@ -127,8 +141,7 @@ def run_stress(steps, warmup, graph_steps=16):
     eng.set_graph_steps(graph_steps)
     eng.set_positions(x)
     eng.step(warmup)
-    brick = eng.autotune(10)
-    shape = {"brick": brick, "defer_depth": eng.defer_depth()}
+    shape = pick_shape(eng, shape, 10)
     t0 = time.perf_counter()
     eng.step(steps)
     dt = time.perf_counter() - t0
@@ -176,6 +189,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stress-steps", type=int, default=1000,
                     help="steps of the N=262144 stress configuration (0 skips it)")
+    ap.add_argument("--shape", default=None,
+                    help="force kernel as BRICK,DEPTH (e.g. 0,16) instead of autotuning")
     ap.add_argument("--graph-steps", type=int, default=16,
                     help="steps per captured HIP graph; 0 launches every kernel eagerly "
                          "(same kernels, for kernel tracing)")
@@ -187,15 +202,16 @@ def main():
         print("bench.py measures one GPU only: replicas shard across GPUs with no per-step "
               "exchange, so N>1 is not run here", file=sys.stderr)
         sys.exit(2)
-    plain = run_plain(a.steps, a.warmup, a.graph_steps)
+    shape = tuple(int(v) for v in a.shape.split(",")) if a.shape else None
+    plain = run_plain(a.steps, a.warmup, a.graph_steps, shape)
     log(f"[bench] plain: {plain['rate']:.4e} particle-steps/s, kernel ms {plain['kernel_ms']}")
     batch = None
     if not a.no_batch:
-        batch = run_batch(a.batch_steps, min(a.warmup, 500), a.graph_steps)
+        batch = run_batch(a.batch_steps, min(a.warmup, 500), a.graph_steps, shape)
         log(f"[bench] batch: {batch['rate']:.4e} particle-steps/s")
     stress = None
     if a.stress_steps > 0:
-        stress = run_stress(a.stress_steps, 100, a.graph_steps)
+        stress = run_stress(a.stress_steps, 100, a.graph_steps, shape)
         log(f"[bench] stress: {stress['rate']:.4e} particle-steps/s")
     cpu = None if a.no_cpu else cpu_baseline(plain)
     N = plain["N"]
@@ -220,7 +236,8 @@ def main():
                    "force_kernel": plain["shape"]},
         "roofline": {"bound": "hbm", "kernel": "k_forces", "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": None, "bytes_per_particle": B_FORCES,
+                     "traffic": TRAFFIC_FORCES_PLAIN, "traffic_unit": "B per launch (rocprofv3)",
+                     "bytes_per_particle": B_FORCES,
                      "avg_launch_ms": f_ms},
         "kernel_ms": {"update": plain["kernel_ms"][0], "cells": plain["kernel_ms"][1],
                       "forces": plain["kernel_ms"][2]},
@@ -239,6 +256,8 @@ def main():
                         "steps": a.batch_steps, "ratio_vs_plain": batch["rate"] / plain["rate"],
                         "step_hbm_frac": B_STEP * batch["rate"] / HBM_PEAK,
                         "kernel_ms": dict(zip(["update", "cells", "forces"], batch["kernel_ms"])),
+                        "forces_hbm_GB_s": B_FORCES * batch["sum_N"] / (batch["kernel_ms"][2] * 1e-3) / 1e9,
+                        "forces_traffic": TRAFFIC_FORCES_BATCH,
                         "force_kernel": batch["shape"]}
     print(json.dumps(out), flush=True)
     if a.copy_calibration:
