@@ -23,6 +23,8 @@ BD_POT_HARMONIC = 2
 BD_ORDER_REFERENCE = 1
 BD_ORDER_SPLIT4 = 4
 BD_ORDER_SPLIT8 = 8
+BD_WALK_ROWS = 0
+BD_WALK_SLABS = 1
 BD_TEMP_LINEAR = 0
 BD_TEMP_SINE = 1
 
@@ -92,6 +94,8 @@ SIGNATURES = {
     "bd_get_force_order": (C.c_int, [_P, _I]),
     "bd_set_graph_steps": (C.c_int, [_P, C.c_int32]),
     "bd_get_graph_steps": (C.c_int, [_P, _I]),
+    "bd_set_walk": (C.c_int, [_P, C.c_int32]),
+    "bd_get_walk": (C.c_int, [_P, _I]),
     "bd_set_defer_depth": (C.c_int, [_P, C.c_int32]),
     "bd_get_defer_depth": (C.c_int, [_P, _I]),
     "bd_set_brick": (C.c_int, [_P, C.c_int32]),

@@ -756,17 +756,157 @@ __device__ inline void scan_rows(const RepDev& R, int n, double cs, double sigma
   if (DEPTH > 0) drain<POT>(R, sigma, x, y, z, cp, buf, nd, fx, fy, fz, vir, pot);
 }
 
+// Slab walk: the same rows, ranges, order and arithmetic as scan_rows, but
+// the candidate loop is flattened over a whole slab (fixed di, all dj):
+// the lane first lists the slab's non-empty slot ranges in LDS (`seg`,
+// stride kBlk: start, then length | y image code << 16 | z image code << 18
+// | centre-row flag << 20), then walks them as one stream.  A wavefront
+// then loops max(sum of the slab's range lengths) times instead of the sum
+// over rows of each row's longest range, which is where the row walk loses
+// most of its lanes.
+constexpr int kSlabSegs = 10;  // 2 ranges x (2 NR + 1) rows, NR <= 2
+
+__device__ inline double image_of(unsigned int code, double L) {
+  return code == 0u ? 0.0 : (code == 1u ? -L : L);
+}
+
+template <int POT, int NR, int DEPTH>
+__device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigma, double rc2,
+                                  int self, double x, double y, double z, const int hw[3],
+                                  const int* __restrict__ rs, const double4* __restrict__ cp,
+                                  unsigned int* buf, unsigned int* seg, double& fx, double& fy,
+                                  double& fz, double& vir, double& pot) {
+  const double L = R.side;
+  const double lim = rc2 + 1e-9;
+  const double ux = x - (double)hw[0] * cs, uy = y - (double)hw[1] * cs,
+               uz = z - (double)hw[2] * cs;
+  double g2lo[NR], g2hi[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    const double lo = slab_gap(-(k + 1), cs, uz);
+    const double hi = slab_gap(k + 1, cs, uz);
+    g2lo[k] = lo * lo;
+    g2hi[k] = hi * hi;
+  }
+  const double selfw = self >= 0 ? (double)self : -1.0;
+  int nd = 0;
+  for (int di = -NR; di <= NR; di++) {
+    const double gx = slab_gap(di, cs, ux);
+    const double gx2 = gx * gx;
+    const double shx = image_shift(hw[0] + di, n, L);
+    int a = hw[0] + di;
+    a = a < 0 ? a + n : (a >= n ? a - n : a);
+    // 1. the slab's ranges
+    int nseg = 0;
+    if (gx2 <= lim) {
+      for (int dj = -NR; dj <= NR; dj++) {
+        const double gy = slab_gap(dj, cs, uy);
+        const double gxy2 = gx2 + gy * gy;
+        if (gxy2 > lim) continue;
+        const unsigned int ycode = image_code(image_shift(hw[1] + dj, n, L));
+        int b = hw[1] + dj;
+        b = b < 0 ? b + n : (b >= n ? b - n : b);
+        int klo = 0, khi = 0;
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+          if (gxy2 + g2lo[k] <= lim) klo = -(k + 1);
+          if (gxy2 + g2hi[k] <= lim) khi = k + 1;
+        }
+        const int* rrow = rs + (a + n * b) * (n + 1);
+        const int z0 = hw[2] + klo, z1 = hw[2] + khi;
+        int s0a, s1a, s0b = 0, s1b = 0;
+        unsigned int za = 0u, zb = 0u;
+        if (z0 < 0) {
+          s0a = rrow[z0 + n];
+          s1a = rrow[n];
+          za = 1u;
+          s0b = rrow[0];
+          s1b = rrow[z1 + 1];
+        } else if (z1 >= n) {
+          s0a = rrow[z0];
+          s1a = rrow[n];
+          s0b = rrow[0];
+          s1b = rrow[z1 - n + 1];
+          zb = 2u;
+        } else {
+          s0a = rrow[z0];
+          s1a = rrow[z1 + 1];
+        }
+        const unsigned int ctr = (self >= 0 && di == 0 && dj == 0) ? (1u << 20) : 0u;
+        if (s1a > s0a) {
+          seg[(2 * nseg) * kBlk] = (unsigned int)s0a;
+          seg[(2 * nseg + 1) * kBlk] = (unsigned int)(s1a - s0a) | (ycode << 16) | (za << 18) | ctr;
+          nseg++;
+        }
+        if (s1b > s0b) {
+          seg[(2 * nseg) * kBlk] = (unsigned int)s0b;
+          seg[(2 * nseg + 1) * kBlk] = (unsigned int)(s1b - s0b) | (ycode << 16) | (zb << 18) | ctr;
+          nseg++;
+        }
+      }
+    }
+    // 2. one stream over the slab's ranges, in listing order; the next
+    //    candidate is loaded before the current one is evaluated
+    const unsigned int xbits = image_code(shx) << kSlotBits;
+    bool more = nseg > 0;
+    int k = 0, t = 0, len = 0, st = 0;
+    unsigned int meta = 0u;
+    double4 qn = make_double4(0.0, 0.0, 0.0, -1.0);
+    if (more) {
+      st = (int)seg[0];
+      meta = seg[kBlk];
+      len = (int)(meta & 0xffffu);
+      k = 1;
+      qn = cp[st];
+    }
+    while (more) {
+      const double4 q = qn;
+      const int sl = st + t;
+      const unsigned int mq = meta;
+      if (++t == len) {
+        if (k < nseg) {
+          st = (int)seg[(2 * k) * kBlk];
+          meta = seg[(2 * k + 1) * kBlk];
+          len = (int)(meta & 0xffffu);
+          t = 0;
+          k++;
+        } else {
+          more = false;
+        }
+      }
+      if (more) qn = cp[st + t];
+      const double dx = (q.x - x) + shx;
+      const double dy = (q.y - y) + image_of((mq >> 16) & 3u, L);
+      const double dz = (q.z - z) + image_of((mq >> 18) & 3u, L);
+      const double dist2 = dx * dx + dy * dy + dz * dz;
+      if (dist2 < rc2 && !((mq >> 20) != 0u && q.w == selfw)) {
+        if (DEPTH == 0) {
+          pair_add<POT>(R, sigma, dist2, dx, dy, dz, fx, fy, fz, vir, pot);
+        } else {
+          buf[nd * kBlk] = (unsigned int)sl | xbits | (((mq >> 16) & 3u) << (kSlotBits + 2)) |
+                           (((mq >> 18) & 3u) << (kSlotBits + 4));
+          if (++nd == DEPTH) {
+            drain<POT>(R, sigma, x, y, z, cp, buf, nd, fx, fy, fz, vir, pot);
+            nd = 0;
+          }
+        }
+      }
+    }
+  }
+  if (DEPTH > 0) drain<POT>(R, sigma, x, y, z, cp, buf, nd, fx, fy, fz, vir, pot);
+}
+
 // Both stencils of replica-local particle i (same species, then the other
 // species, main.cu:576-585) from global memory, as one rolled pass loop.
 // SPLIT walks cells (scan_fast) so that G lanes can share the stencil; the
 // single-lane walk uses row ranges (scan_rows), deferring pair terms through
 // `buf` when DEPTH > 0.
-template <int POT, bool SPLIT, int DEPTH = 0>
+template <int POT, bool SPLIT, int DEPTH = 0, int WALK = 0>
 __device__ inline void forces_global(const RepDev& R, int i, double x, double y, double z,
                                      const int* __restrict__ rs,
                                      const double4* __restrict__ cpos, double& fx, double& fy,
                                      double& fz, double& v, double& u, int sub = 0, int G = 1,
-                                     unsigned int* buf = nullptr) {
+                                     unsigned int* buf = nullptr, unsigned int* seg = nullptr) {
   const int g = i < R.nb ? 0 : 1;
   int cbase = 0;
 #pragma unroll
@@ -789,6 +929,13 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
         scan_fast_nr<POT, true>(nr, R, n, cs, sigma, rc2, self, x, y, z, hg, hg, acc, fx, fy,
                                 fz, v, u, sub, G, cbase);
         cbase += (2 * nr + 1) * (2 * nr + 1) * (2 * nr + 1);
+      } else if (WALK == 1) {
+        if (nr == 1)
+          scan_slabs<POT, 1, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, buf, seg,
+                                    fx, fy, fz, v, u);
+        else
+          scan_slabs<POT, 2, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, buf, seg,
+                                    fx, fy, fz, v, u);
       } else if (nr == 1) {
         scan_rows<POT, 1, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, buf, fx,
                                  fy, fz, v, u);
@@ -807,8 +954,9 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
 // Forces, virials and potentials of every particle: same-species stencil
 // then cross-species stencil (main.cu:576-585), accumulators start at zero.
 // NEIGH records the interacting partners in traversal order (test hook).
-// DEPTH: per-thread LDS entries for deferred pair terms (0 = direct).
-template <int POT, bool NEIGH, int DEPTH>
+// DEPTH: per-thread LDS entries for deferred pair terms (0 = direct);
+// WALK: 0 row walk, 1 slab walk.
+template <int POT, bool NEIGH, int DEPTH, int WALK>
 __global__ void __launch_bounds__(kBlk)
     k_forces(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
              const double* __restrict__ px, const double* __restrict__ py,
@@ -818,6 +966,7 @@ __global__ void __launch_bounds__(kBlk)
              double* __restrict__ pot, int* __restrict__ ncounts, int* __restrict__ nlist,
              int maxn) {
   __shared__ unsigned int dbuf[DEPTH > 0 ? DEPTH * kBlk : 1];
+  __shared__ unsigned int sbuf[WALK == 1 ? 2 * kSlabSegs * kBlk : 1];
   const int4 t = tbl[blockIdx.x];
   if (t.x < 0) return;
   const int i = t.y + (int)threadIdx.x;
@@ -836,8 +985,8 @@ __global__ void __launch_bounds__(kBlk)
                          cpos, fx, fy, fz, v, u, nl, nc, maxn);
     ncounts[i] = nc;
   } else {
-    forces_global<POT, false, DEPTH>(R, i, x, y, z, rs, cpos, fx, fy, fz, v, u, 0, 1,
-                                     dbuf + threadIdx.x);
+    forces_global<POT, false, DEPTH, WALK>(R, i, x, y, z, rs, cpos, fx, fy, fz, v, u, 0, 1,
+                                           dbuf + threadIdx.x, sbuf + threadIdx.x);
   }
   ffx[p] = fx;
   ffy[p] = fy;
@@ -1393,8 +1542,10 @@ struct bd_sys {
   int4* tbl_brick = nullptr;
   // force summation order: 1 = reference (bit-exact), G = 4 or 8 = split
   int order = 1;
-  // deferred pair terms of the row walk: 0 (direct), 16 or 32 LDS entries
+  // deferred pair terms of the walk: 0 (direct), 16 or 32 LDS entries
   int defer = 16;
+  // walk of the default force kernel: 0 rows, 1 slabs
+  int walk = 0;
   bool defer_ok = true;  // every grid's slot index fits the 26-bit record
   std::vector<int4> tbl_split_h;
   int4* tbl_split = nullptr;
@@ -1426,35 +1577,45 @@ void launch_cells(bd_sys* s, int advance) {
                                       s->draws, s->R, advance);
 }
 
-template <bool NEIGH, int DEPTH>
+template <bool NEIGH, int DEPTH, int WALK>
 void launch_forces_k(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl, int nblk) {
   switch (s->pot_kind) {
     case BD_POT_WCA:
-      k_forces<BD_POT_WCA, NEIGH, DEPTH><<<nblk, kBlk, 0, s->stream>>>(
+      k_forces<BD_POT_WCA, NEIGH, DEPTH, WALK><<<nblk, kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, ncounts, nlist, maxn);
       break;
     case BD_POT_WCA_MODIFIED:
-      k_forces<BD_POT_WCA_MODIFIED, NEIGH, DEPTH><<<nblk, kBlk, 0, s->stream>>>(
+      k_forces<BD_POT_WCA_MODIFIED, NEIGH, DEPTH, WALK><<<nblk, kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, ncounts, nlist, maxn);
       break;
     default:
-      k_forces<BD_POT_HARMONIC, NEIGH, DEPTH><<<nblk, kBlk, 0, s->stream>>>(
+      k_forces<BD_POT_HARMONIC, NEIGH, DEPTH, WALK><<<nblk, kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, ncounts, nlist, maxn);
       break;
   }
 }
 
+template <int WALK>
+void launch_forces_w(bd_sys* s, const int4* tbl, int nblk) {
+  if (s->defer == 0)
+    launch_forces_k<false, 0, WALK>(s, nullptr, nullptr, 0, tbl, nblk);
+  else if (s->defer == 16)
+    launch_forces_k<false, 16, WALK>(s, nullptr, nullptr, 0, tbl, nblk);
+  else
+    launch_forces_k<false, 32, WALK>(s, nullptr, nullptr, 0, tbl, nblk);
+}
+
 template <bool NEIGH>
 void launch_forces_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl, int nblk) {
-  if (NEIGH || s->defer == 0)
-    launch_forces_k<NEIGH, 0>(s, ncounts, nlist, maxn, tbl, nblk);
-  else if (s->defer == 16)
-    launch_forces_k<false, 16>(s, ncounts, nlist, maxn, tbl, nblk);
+  if (NEIGH)
+    launch_forces_k<true, 0, 0>(s, ncounts, nlist, maxn, tbl, nblk);
+  else if (s->walk == 1)
+    launch_forces_w<1>(s, tbl, nblk);
   else
-    launch_forces_k<false, 32>(s, ncounts, nlist, maxn, tbl, nblk);
+    launch_forces_w<0>(s, tbl, nblk);
 }
 
 template <int CAP>
@@ -2254,13 +2415,14 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
     if (FILE* f = std::fopen(cache_path, "r")) {
       char k[200];
       int b, d;
-      int found_b = -1, found_d = 0;
+      int found_b = -2, found_d = 0;
       while (std::fscanf(f, "%199s %d %d", k, &b, &d) == 3)
         if (key == k) found_b = b, found_d = d;
       std::fclose(f);
-      if (found_b >= 0 && bd_set_defer_depth(s, found_d) == BD_OK &&
-          bd_set_brick(s, found_b) == BD_OK) {
-        if (chosen) *chosen = found_b;
+      if (found_b >= -1 && bd_set_defer_depth(s, found_d) == BD_OK &&
+          bd_set_walk(s, found_b < 0 ? BD_WALK_SLABS : BD_WALK_ROWS) == BD_OK &&
+          bd_set_brick(s, found_b < 0 ? 0 : found_b) == BD_OK) {
+        if (chosen) *chosen = found_b < 0 ? 0 : found_b;
         return BD_OK;
       }
     }
@@ -2280,19 +2442,22 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
   }
   // kernel shapes: the row walk at three deferral depths, the LDS bricks
   struct Shape {
-    int brick, defer;
+    int brick, defer;  // brick -1: the slab walk (brick 0 with walk 1)
   };
-  const Shape candidates[] = {{0, 0}, {0, 16}, {0, 32}, {1, 0}, {2, 0}, {3, 0}, {4, 0}};
+  const Shape candidates[] = {{0, 0},  {0, 16},  {0, 32},  {-1, 0}, {-1, 16},
+                              {-1, 32}, {1, 0},   {2, 0},   {3, 0},   {4, 0}};
   Shape best{0, 0};
   double best_ms = 1e300;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  const Shape keep{s->brick, s->defer};
+  const Shape keep{s->walk == 1 ? -1 : s->brick, s->defer};
   for (const Shape& c : candidates) {
-    if (s->order != BD_ORDER_REFERENCE && (c.brick != 0 || c.defer != keep.defer)) continue;
+    if (s->order != BD_ORDER_REFERENCE && (c.brick != keep.brick || c.defer != keep.defer))
+      continue;
     if (c.defer != 0 && !s->defer_ok) continue;
-    if (setup_bricks(s, c.brick) != BD_OK) continue;
+    if (setup_bricks(s, c.brick < 0 ? 0 : c.brick) != BD_OK) continue;
+    s->walk = c.brick < 0 ? 1 : 0;
     s->defer = c.defer;
     launch_forces(s);  // warm-up
     HIP_TRY(hipEventRecord(e0, s->stream));
@@ -2310,10 +2475,12 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
   (void)hipEventDestroy(e1);
   g_err.clear();
   s->defer = best.defer;
-  rc = bd_set_brick(s, best.brick);
+  s->walk = best.brick < 0 ? 1 : 0;
+  rc = bd_set_brick(s, best.brick < 0 ? 0 : best.brick);
   if (rc != BD_OK) {
     s->defer = keep.defer;
-    bd_set_brick(s, keep.brick);
+    s->walk = keep.brick < 0 ? 1 : 0;
+    bd_set_brick(s, keep.brick < 0 ? 0 : keep.brick);
     return rc;
   }
   if (cache_path)
@@ -2321,7 +2488,27 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
       std::fprintf(f, "%s %d %d\n", key.c_str(), best.brick, best.defer);
       std::fclose(f);
     }
-  if (chosen) *chosen = best.brick;
+  if (chosen) *chosen = best.brick < 0 ? 0 : best.brick;
+  return BD_OK;
+}
+
+int bd_set_walk(bd_sys* s, int32_t walk) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (walk != BD_WALK_ROWS && walk != BD_WALK_SLABS)
+    return fail(BD_E_ARG, "walk must be BD_WALK_ROWS or BD_WALK_SLABS");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->graph) {
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_steps = 0;
+  }
+  s->walk = walk;
+  return BD_OK;
+}
+
+int bd_get_walk(const bd_sys* s, int32_t* walk) {
+  if (!s || !walk) return fail(BD_E_ARG, "null argument");
+  *walk = s->walk;
   return BD_OK;
 }
 

@@ -197,6 +197,15 @@ class Engine:
         check(self.lib.bd_get_graph_steps(self.h, C.byref(v)))
         return v.value
 
+    def set_walk(self, walk):
+        """Force walk: abi.BD_WALK_ROWS (0) or abi.BD_WALK_SLABS (1)."""
+        check(self.lib.bd_set_walk(self.h, int(walk)))
+
+    def walk(self):
+        v = C.c_int32()
+        check(self.lib.bd_get_walk(self.h, C.byref(v)))
+        return v.value
+
     def set_defer_depth(self, depth):
         """Deferred pair terms of the row walk: 0, 16 or 32 LDS entries."""
         check(self.lib.bd_set_defer_depth(self.h, int(depth)))

@@ -59,8 +59,10 @@ def pick_shape(eng, shape, trials):
     else:
         eng.set_brick(shape[0])
         eng.set_defer_depth(shape[1])
+        eng.set_walk(shape[2] if len(shape) > 2 else 0)
         brick = shape[0]
-    return {"brick": brick, "defer_depth": eng.defer_depth(),
+    return {"brick": brick, "walk": ["rows", "slabs"][eng.walk()],
+            "defer_depth": eng.defer_depth(),
             "chosen_by": "autotuner" if shape is None else "--shape"}
 
 
@@ -190,7 +192,7 @@ def main():
     ap.add_argument("--stress-steps", type=int, default=1000,
                     help="steps of the N=262144 stress configuration (0 skips it)")
     ap.add_argument("--shape", default=None,
-                    help="force kernel as BRICK,DEPTH (e.g. 0,16) instead of autotuning")
+                    help="force kernel as BRICK,DEPTH[,WALK] (e.g. 0,16,1) instead of autotuning")
     ap.add_argument("--graph-steps", type=int, default=16,
                     help="steps per captured HIP graph; 0 launches every kernel eagerly "
                          "(same kernels, for kernel tracing)")

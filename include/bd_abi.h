@@ -158,13 +158,23 @@ int bd_get_brick(const bd_sys* s, int32_t* brick);
 int bd_set_defer_depth(bd_sys* s, int32_t depth);
 int bd_get_defer_depth(const bd_sys* s, int32_t* depth);
 
+/* Walk of the thread-per-particle force kernel (no reference counterpart):
+ * BD_WALK_ROWS (default) loops over each stencil row's slot range;
+ * BD_WALK_SLABS lists the ranges of all rows with one di in LDS and loops
+ * over them as one stream (fewer idle lanes when range lengths differ
+ * between particles).  Same pairs, order and arithmetic: bit-identical. */
+enum { BD_WALK_ROWS = 0, BD_WALK_SLABS = 1 };
+int bd_set_walk(bd_sys* s, int32_t walk);
+int bd_get_walk(const bd_sys* s, int32_t* walk);
+
 /* Autotuner (no reference counterpart): measures the force kernel for every
- * shape (row walk at deferral depths 0, 16 and 32; bricks 1..4; all
- * bit-identical by construction) with forces-only launches on the current
- * state, keeps the fastest and returns its brick in *chosen (the depth is
- * read with bd_get_defer_depth).  With cache_path != NULL the choice is
- * looked up in / appended to that text file as "key brick depth", keyed by
- * (replicas, N, phi, potential, order). */
+ * shape (row and slab walks at deferral depths 0, 16 and 32; bricks 1..4;
+ * all bit-identical by construction) with forces-only launches on the
+ * current state, keeps the fastest and returns its brick in *chosen (the
+ * walk and depth are read with bd_get_walk / bd_get_defer_depth).  With
+ * cache_path != NULL the choice is looked up in / appended to that text
+ * file as "key brick depth" (brick -1 = slab walk), keyed by (replicas, N,
+ * phi, potential, order). */
 int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32_t* chosen);
 
 /* Force summation order (no reference counterpart).

@@ -136,17 +136,24 @@ def test_every_brick_shape_bit_identical(config1_positions, pot):
             continue
         tried.append(b)
         assert e.brick() == b
-        for depth in ((0, 16, 32) if b == 0 else (16,)):
-            e.set_defer_depth(depth)
-            assert e.defer_depth() == depth
-            e.compute_forces()
-            fe, ve, ue = e.forces()
-            assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and \
-                np.array_equal(ue, uo), (b, depth)
+        for walk in ((abi.BD_WALK_ROWS, abi.BD_WALK_SLABS) if b == 0 else (abi.BD_WALK_ROWS,)):
+            e.set_walk(walk)
+            assert e.walk() == walk
+            for depth in ((0, 16, 32) if b == 0 else (16,)):
+                e.set_defer_depth(depth)
+                assert e.defer_depth() == depth
+                e.compute_forces()
+                fe, ve, ue = e.forces()
+                assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and \
+                    np.array_equal(ue, uo), (b, walk, depth)
+        e.set_walk(abi.BD_WALK_ROWS)
     assert {0, 1, 2, 3} <= set(tried)
     for bad in (-1, 8, 64):
         with pytest.raises(BDError):
             e.set_defer_depth(bad)
+    for bad in (-1, 2):
+        with pytest.raises(BDError):
+            e.set_walk(bad)
     e.close()
 
 
@@ -169,14 +176,16 @@ def test_forces_bit_exact_at_bench_size(plain_positions):
     e = Engine(p)
     e.set_positions(x)
     e.cell_locate()
-    shapes = [(0, 0), (0, 16), (0, 32), (1, 16), (2, 16)]
-    for b, depth in shapes:
+    shapes = [(0, 0, 0), (0, 16, 0), (0, 32, 0), (0, 0, 1), (0, 16, 1), (0, 32, 1), (1, 16, 0),
+              (2, 16, 0)]
+    for b, depth, walk in shapes:
         e.set_brick(b)
         e.set_defer_depth(depth)
+        e.set_walk(walk)
         e.compute_forces()
         fe, ve, ue = e.forces()
         assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and np.array_equal(ue, uo), \
-            (b, depth)
+            (b, depth, walk)
     for sp in (0, 1):
         ne, ce = e.cells(sp)
         no, co = o.cells(sp)
@@ -426,7 +435,8 @@ def test_autotune_keeps_results_and_caches(config1_positions, tmp_path):
     f1 = e.forces()
     assert all(np.array_equal(a, c) for a, c in zip(f0, f1))
     last = open(cache).read().split()
-    assert last[-2] == str(b) and last[-1] == str(e.defer_depth())
+    assert last[-2] == str(-1 if e.walk() == abi.BD_WALK_SLABS else b)
+    assert last[-1] == str(e.defer_depth())
     e2 = Engine(p)
     e2.set_positions(x)
     assert e2.autotune(5, cache) == b  # served from the cache
