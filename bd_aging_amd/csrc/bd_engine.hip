@@ -606,33 +606,45 @@ __device__ inline void pair_add(const RepDev& R, double sigma, double dist2, dou
 }
 
 // Deferred pair terms: the walk records each partner inside the cutoff as
-// slot | image codes (2 bits per axis: 0 none, 1 -L, 2 +L) in a per-thread
-// LDS column; drain() evaluates the recorded terms in recording order,
-// recomputing the separation with the same operations.  The potential is
-// then evaluated by all lanes together on interacting pairs only, instead of
-// by the whole wavefront whenever one lane meets a partner.
-constexpr unsigned int kSlotBits = 26;
+// slot | image codes (2 bits per axis: 0 none, 1 -L, 2 +L) | pass bit in a
+// per-thread LDS column, for both stencil passes of the particle; drain()
+// evaluates the recorded terms in recording order, recomputing the
+// separation with the same operations.  The potential is then evaluated by
+// all lanes together on interacting pairs only, instead of by the whole
+// wavefront whenever one lane meets a partner.
+constexpr unsigned int kSlotBits = 25;
 constexpr unsigned int kSlotMask = (1u << kSlotBits) - 1u;
+constexpr unsigned int kPassBit = 1u << 31;
 
 __device__ inline unsigned int image_code(double sh) {
   return sh == 0.0 ? 0u : (sh < 0.0 ? 1u : 2u);
 }
 
+struct Defer {
+  const double4* cp0;  // slots of the same-species pass (pass bit 0)
+  const double4* cp1;  // slots of the cross-species pass (pass bit 1)
+  double sigma0, sigma1;
+  unsigned int* buf;   // LDS column, stride kBlk
+  int nd;
+};
+
 template <int POT>
-__device__ inline void drain(const RepDev& R, double sigma, double x, double y, double z,
-                             const double4* __restrict__ cp, const unsigned int* buf, int nd,
+__device__ inline void drain(const RepDev& R, double x, double y, double z, Defer& d,
                              double& fx, double& fy, double& fz, double& vir, double& pot) {
   const double L = R.side;
+  const int nd = d.nd;
+  d.nd = 0;
   for (int e = 0; e < nd; e++) {
-    const unsigned int w = buf[e * kBlk];
-    const double4 q = cp[w & kSlotMask];
-    const unsigned int cx = (w >> kSlotBits) & 3u, cy = (w >> (kSlotBits + 2)) & 3u,
-                       cz = w >> (kSlotBits + 4);
-    const double dx = (q.x - x) + (cx == 0u ? 0.0 : (cx == 1u ? -L : L));
-    const double dy = (q.y - y) + (cy == 0u ? 0.0 : (cy == 1u ? -L : L));
-    const double dz = (q.z - z) + (cz == 0u ? 0.0 : (cz == 1u ? -L : L));
+    const unsigned int wc = d.buf[e * kBlk];
+    const double4 qc = ((wc & kPassBit) ? d.cp1 : d.cp0)[wc & kSlotMask];
+    const unsigned int cx = (wc >> kSlotBits) & 3u, cy = (wc >> (kSlotBits + 2)) & 3u,
+                       cz = (wc >> (kSlotBits + 4)) & 3u;
+    const double dx = (qc.x - x) + (cx == 0u ? 0.0 : (cx == 1u ? -L : L));
+    const double dy = (qc.y - y) + (cy == 0u ? 0.0 : (cy == 1u ? -L : L));
+    const double dz = (qc.z - z) + (cz == 0u ? 0.0 : (cz == 1u ? -L : L));
     const double dist2 = dx * dx + dy * dy + dz * dz;
-    pair_add<POT>(R, sigma, dist2, dx, dy, dz, fx, fy, fz, vir, pot);
+    pair_add<POT>(R, (wc & kPassBit) ? d.sigma1 : d.sigma0, dist2, dx, dy, dz, fx, fy, fz, vir,
+                  pot);
   }
 }
 
@@ -649,8 +661,8 @@ template <int POT, int NR, int DEPTH>
 __device__ inline void scan_rows(const RepDev& R, int n, double cs, double sigma, double rc2,
                                  int self, double x, double y, double z, const int hw[3],
                                  const int* __restrict__ rs, const double4* __restrict__ cp,
-                                 unsigned int* buf, double& fx, double& fy, double& fz,
-                                 double& vir, double& pot) {
+                                 Defer& df, unsigned int passbit, double& fx, double& fy,
+                                 double& fz, double& vir, double& pot) {
   const double L = R.side;
   const double lim = rc2 + 1e-9;
   const double ux = x - (double)hw[0] * cs, uy = y - (double)hw[1] * cs,
@@ -663,7 +675,6 @@ __device__ inline void scan_rows(const RepDev& R, int n, double cs, double sigma
     g2lo[k] = lo * lo;
     g2hi[k] = hi * hi;
   }
-  int nd = 0;
   for (int di = -NR; di <= NR; di++) {
     const double gx = slab_gap(di, cs, ux);
     const double gx2 = gx * gx;
@@ -743,17 +754,13 @@ __device__ inline void scan_rows(const RepDev& R, int n, double cs, double sigma
           const double dz = (q.z - z) + (in_a ? sha : shb);
           const double dist2 = dx * dx + dy * dy + dz * dz;
           if (dist2 < rc2 && q.w != selfw) {
-            buf[nd * kBlk] = (unsigned int)sl | (in_a ? ca : cb);
-            if (++nd == DEPTH) {
-              drain<POT>(R, sigma, x, y, z, cp, buf, nd, fx, fy, fz, vir, pot);
-              nd = 0;
-            }
+            df.buf[df.nd * kBlk] = (unsigned int)sl | (in_a ? ca : cb) | passbit;
+            if (++df.nd == DEPTH) drain<POT>(R, x, y, z, df, fx, fy, fz, vir, pot);
           }
         }
       }
     }
   }
-  if (DEPTH > 0) drain<POT>(R, sigma, x, y, z, cp, buf, nd, fx, fy, fz, vir, pot);
 }
 
 // Slab walk: the same rows, ranges, order and arithmetic as scan_rows, but
@@ -774,8 +781,8 @@ template <int POT, int NR, int DEPTH>
 __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigma, double rc2,
                                   int self, double x, double y, double z, const int hw[3],
                                   const int* __restrict__ rs, const double4* __restrict__ cp,
-                                  unsigned int* buf, unsigned int* seg, double& fx, double& fy,
-                                  double& fz, double& vir, double& pot) {
+                                  Defer& df, unsigned int passbit, unsigned int* seg, double& fx,
+                                  double& fy, double& fz, double& vir, double& pot) {
   const double L = R.side;
   const double lim = rc2 + 1e-9;
   const double ux = x - (double)hw[0] * cs, uy = y - (double)hw[1] * cs,
@@ -789,7 +796,6 @@ __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigm
     g2hi[k] = hi * hi;
   }
   const double selfw = self >= 0 ? (double)self : -1.0;
-  int nd = 0;
   for (int di = -NR; di <= NR; di++) {
     const double gx = slab_gap(di, cs, ux);
     const double gx2 = gx * gx;
@@ -883,17 +889,14 @@ __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigm
         if (DEPTH == 0) {
           pair_add<POT>(R, sigma, dist2, dx, dy, dz, fx, fy, fz, vir, pot);
         } else {
-          buf[nd * kBlk] = (unsigned int)sl | xbits | (((mq >> 16) & 3u) << (kSlotBits + 2)) |
-                           (((mq >> 18) & 3u) << (kSlotBits + 4));
-          if (++nd == DEPTH) {
-            drain<POT>(R, sigma, x, y, z, cp, buf, nd, fx, fy, fz, vir, pot);
-            nd = 0;
-          }
+          df.buf[df.nd * kBlk] = (unsigned int)sl | xbits |
+                                 (((mq >> 16) & 3u) << (kSlotBits + 2)) |
+                                 (((mq >> 18) & 3u) << (kSlotBits + 4)) | passbit;
+          if (++df.nd == DEPTH) drain<POT>(R, x, y, z, df, fx, fy, fz, vir, pot);
         }
       }
     }
   }
-  if (DEPTH > 0) drain<POT>(R, sigma, x, y, z, cp, buf, nd, fx, fy, fz, vir, pot);
 }
 
 // Both stencils of replica-local particle i (same species, then the other
@@ -909,6 +912,9 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
                                      unsigned int* buf = nullptr, unsigned int* seg = nullptr) {
   const int g = i < R.nb ? 0 : 1;
   int cbase = 0;
+  Defer df{cpos + (g == 0 ? R.tag_off[0] : R.tag_off[1]),
+           cpos + (g == 0 ? R.tag_off[1] : R.tag_off[0]), g == 0 ? R.sigma[0] : R.sigma[1],
+           R.sigma_bs, buf, 0};
 #pragma unroll
   for (int pass = 0; pass < 2; pass++) {
     const int gg = pass == 0 ? g : 1 - g;
@@ -931,17 +937,17 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
         cbase += (2 * nr + 1) * (2 * nr + 1) * (2 * nr + 1);
       } else if (WALK == 1) {
         if (nr == 1)
-          scan_slabs<POT, 1, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, buf, seg,
-                                    fx, fy, fz, v, u);
+          scan_slabs<POT, 1, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df,
+                                    pass == 0 ? 0u : kPassBit, seg, fx, fy, fz, v, u);
         else
-          scan_slabs<POT, 2, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, buf, seg,
-                                    fx, fy, fz, v, u);
+          scan_slabs<POT, 2, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df,
+                                    pass == 0 ? 0u : kPassBit, seg, fx, fy, fz, v, u);
       } else if (nr == 1) {
-        scan_rows<POT, 1, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, buf, fx,
-                                 fy, fz, v, u);
+        scan_rows<POT, 1, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df,
+                                 pass == 0 ? 0u : kPassBit, fx, fy, fz, v, u);
       } else {
-        scan_rows<POT, 2, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, buf, fx,
-                                 fy, fz, v, u);
+        scan_rows<POT, 2, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df,
+                                 pass == 0 ? 0u : kPassBit, fx, fy, fz, v, u);
       }
     } else {
       int nc = 0;
@@ -949,6 +955,7 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
                             nullptr, nc, 0);
     }
   }
+  if (DEPTH > 0 && !SPLIT) drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
 }
 
 // Forces, virials and potentials of every particle: same-species stencil
@@ -1545,7 +1552,7 @@ struct bd_sys {
   // deferred pair terms of the walk: 0 (direct), 16 or 32 LDS entries
   int defer = 16;
   // walk of the default force kernel: 0 rows, 1 slabs
-  int walk = 0;
+  int walk = 1;
   bool defer_ok = true;  // every grid's slot index fits the 26-bit record
   std::vector<int4> tbl_split_h;
   int4* tbl_split = nullptr;

@@ -129,7 +129,7 @@ def run_batch(steps, warmup, graph_steps=16, shape=None):
     eng.close()
     tot = int(n.sum())
     return {"replicas": len(n), "sum_N": tot, "seconds": dt, "rate": tot * steps / dt,
-            "kernel_ms": ms.tolist(), "shape": shape}
+            "kernel_ms": ms.tolist(), "shape": shape, "specs": specs, "T": T, "x": xs}
 
 
 def run_stress(steps, warmup, graph_steps=16, shape=None):
@@ -181,6 +181,37 @@ def cpu_baseline(plain, budget_s=10.0):
                       f"({dt:.1f} s, one thread)"}
 
 
+def _oracle_replica(task):
+    """One replica of the batch on the CPU oracle (a pool worker)."""
+    nb, ns, side, T, seed, x, steps = task
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    o = orc.System(make_params(nb, ns, side=side, dt=1e-4, T0=T, Tf=T, seed=seed))
+    o.set_positions(x)
+    o.cell_locate()
+    o.compute_forces()
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return (nb + ns) * steps, dt
+
+
+def cpu_baseline_batch(pool, workers, batch, replicas=64, steps=150):
+    """The oracle over the first `replicas` replicas of the batch, one replica
+    per task on a pool of host processes (all the cores given to the job)."""
+    tasks = [(nb, ns, side, float(batch["T"][r]), 90000 + r, batch["x"][r], steps)
+             for r, (nb, ns, side, _, _, _) in enumerate(batch["specs"][:replicas])]
+    t0 = time.perf_counter()
+    out = pool.map(_oracle_replica, tasks, chunksize=1)
+    wall = time.perf_counter() - t0
+    work = sum(w for w, _ in out)
+    return {"value": work / wall, "unit": "particle-steps/s", "cores": workers, "kind": "port",
+            "sample": f"FP64 oracle, first {replicas} replicas of the 256-replica batch "
+                      f"(sum N = {sum(t[0] + t[1] for t in tasks)}), {steps} steps each, one "
+                      f"process per replica on {workers} host processes ({wall:.1f} s)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +236,12 @@ def main():
               "exchange, so N>1 is not run here", file=sys.stderr)
         sys.exit(2)
     shape = tuple(int(v) for v in a.shape.split(",")) if a.shape else None
+    pool, workers = None, 0
+    if not a.no_cpu and not a.no_batch:
+        # forked before the GPU is touched: the workers run only the CPU oracle
+        import multiprocessing as mp
+        workers = max(1, min(16, os.cpu_count() or 1))
+        pool = mp.get_context("fork").Pool(workers)
     plain = run_plain(a.steps, a.warmup, a.graph_steps, shape)
     log(f"[bench] plain: {plain['rate']:.4e} particle-steps/s, kernel ms {plain['kernel_ms']}")
     batch = None
@@ -216,6 +253,12 @@ def main():
         stress = run_stress(a.stress_steps, 100, a.graph_steps, shape)
         log(f"[bench] stress: {stress['rate']:.4e} particle-steps/s")
     cpu = None if a.no_cpu else cpu_baseline(plain)
+    cpu_batch = None
+    if pool is not None and batch:
+        cpu_batch = cpu_baseline_batch(pool, workers, batch)
+        pool.close()
+        pool.join()
+        log(f"[bench] cpu batch baseline: {cpu_batch['value']:.4e} particle-steps/s")
     N = plain["N"]
     f_ms = plain["kernel_ms"][2]
     achieved = B_FORCES * N / (f_ms * 1e-3)
@@ -260,6 +303,7 @@ def main():
                         "kernel_ms": dict(zip(["update", "cells", "forces"], batch["kernel_ms"])),
                         "forces_hbm_GB_s": B_FORCES * batch["sum_N"] / (batch["kernel_ms"][2] * 1e-3) / 1e9,
                         "forces_traffic": TRAFFIC_FORCES_BATCH,
+                        "cpu_baseline": cpu_batch,
                         "force_kernel": batch["shape"]}
     print(json.dumps(out), flush=True)
     if a.copy_calibration:

@@ -159,10 +159,10 @@ int bd_set_defer_depth(bd_sys* s, int32_t depth);
 int bd_get_defer_depth(const bd_sys* s, int32_t* depth);
 
 /* Walk of the thread-per-particle force kernel (no reference counterpart):
- * BD_WALK_ROWS (default) loops over each stencil row's slot range;
- * BD_WALK_SLABS lists the ranges of all rows with one di in LDS and loops
- * over them as one stream (fewer idle lanes when range lengths differ
- * between particles).  Same pairs, order and arithmetic: bit-identical. */
+ * BD_WALK_ROWS loops over each stencil row's slot range; BD_WALK_SLABS
+ * (default) lists the ranges of all rows with one di in LDS and loops over
+ * them as one stream (fewer idle lanes when range lengths differ between
+ * particles).  Same pairs, order and arithmetic: bit-identical. */
 enum { BD_WALK_ROWS = 0, BD_WALK_SLABS = 1 };
 int bd_set_walk(bd_sys* s, int32_t walk);
 int bd_get_walk(const bd_sys* s, int32_t* walk);
