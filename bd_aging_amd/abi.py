@@ -101,6 +101,8 @@ SIGNATURES = {
     "bd_set_brick": (C.c_int, [_P, C.c_int32]),
     "bd_get_brick": (C.c_int, [_P, _I]),
     "bd_kernel_times": (C.c_int, [_P, C.c_int32, _D]),
+    "bd_checkpoint_save": (C.c_int, [_P, C.c_char_p]),
+    "bd_checkpoint_load": (C.c_int, [_P, C.c_char_p]),
     "bd_get_rng_lane": (C.c_int, [_P, C.c_int32, C.c_int32, _U, _U, _U]),
     "bd_rng_stream": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, _U]),
     "bd_lane_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),

@@ -6,6 +6,14 @@
 //       wca_aging_linear.data is read first when present, main.cu:90-91),
 //       ../configs/[decadeK/]init_config_i inputs and results/[decadeK/]*.out
 //       outputs (main.cu:885-980).
+//   bd_aging_self --aging < bd_aging_self.data
+//       the same run; with log sampling and decade dumps it also measures F_s
+//       and the MSD from every dumped waiting time inside this one run and
+//       writes them to results/decadeK/ (lag, value), in place of one
+//       reheat-and-restart run per decade (run-fself.sh:149-197).
+//   bd_aging_self --system < bd_aging.data
+//       the effective-force workflow of run-system.sh:118-145: aging run plus
+//       the Contracting_Vector3D force contraction, results/force_contraction.out.
 //   bd_aging_self --make-configs NBIG NSML PHI SIGMA_BIG SIGMA_SML NCONFIGS SEED DIR
 //                 [T EQ_STEPS DT]
 //       writes DIR/init_config_1..NCONFIGS: random placement, overlap removal
@@ -255,7 +263,8 @@ int make_configs(int argc, char** argv) {
     bd_destroy(s);
     std::printf("equilibration: %d steps at T=%g\n", eq_steps, T);
   }
-  mkdir(dir.c_str(), 0755);
+  for (size_t k = 1; k <= dir.size(); k++)  // every missing level, as mkdir -p
+    if (k == dir.size() || dir[k] == '/') mkdir(dir.substr(0, k).c_str(), 0755);
   InitConfig ic;
   ic.n_big = nb;
   ic.n_sml = ns;
@@ -276,8 +285,195 @@ int make_configs(int argc, char** argv) {
 
 }  // namespace
 
+// g(r) normalisation (main.cu:849-877) and gder_k.out files (main.cu:885-900).
+void write_gders(const std::string& dir, std::vector<double>& gders, int nsamples, int nbins,
+                 double range, const InitConfig& c0, int n_configs) {
+  const double bin = range / (double)nbins;
+  const double volume = (double)c0.side * c0.side * c0.side;
+  const double xb = c0.n_big, xs = c0.n_sml;
+  const double sbs = 0.5 * ((double)c0.sigma_big + (double)c0.sigma_sml);
+  auto vfree = [&](double sg) { return volume - (4.0 / 3.0) * kPiRef * sg * sg * sg; };
+  for (int k = 0; k < nsamples; k++) {
+    double* gk = &gders[(size_t)k * 4 * nbins];
+    for (int nb = 0; nb < nbins; nb++) {
+      const double xnb = nb;
+      const double shell = 4.0 * kPiRef * bin * bin * bin * ((1.0 / 3.0) + xnb * xnb + xnb);
+      gk[nb] = gk[nb] / shell / (xb * (xb - 1.0) / vfree(c0.sigma_big)) / n_configs;
+      gk[nbins + nb] = gk[nbins + nb] / shell / (xb * xs / vfree(sbs)) / n_configs;
+      gk[2 * nbins + nb] = gk[2 * nbins + nb] / shell / (xs * xb / vfree(sbs)) / n_configs;
+      gk[3 * nbins + nb] = gk[3 * nbins + nb] / shell / (xs * (xs - 1.0) / vfree(c0.sigma_sml)) /
+                           n_configs;
+    }
+    FILE* f = std::fopen((dir + "gder_" + std::to_string(k) + ".out").c_str(), "w");
+    if (!f) die("Verify file path: " + dir, -2);
+    for (int nb = 0; nb < nbins; nb++)
+      std::fprintf(f, "%f  %f  %f  %f  %f\n", (0.5 + nb) * bin, gk[nb], gk[nbins + nb],
+                   gk[2 * nbins + nb], gk[3 * nbins + nb]);
+    std::fclose(f);
+  }
+}
+
+// The effective-force workflow of run-system.sh:118-145 (its bd_aging.bin is
+// not part of the reference): the same protocol as the aging run with the
+// configs directory line and a force_nbins / force_range line, the aging run
+// of main.cu, and the Contracting_Vector3D force contraction
+// (Contracting_Vector3D.cu:11-304) accumulated over every sample of every
+// config on the GPU.  Writes results/gder_k.out, energies_averaged.out,
+// pressure.out and force_contraction.out (bin centre, contracted force).
+int run_system() {
+  FILE* fp = std::fopen("bd_aging.data", "r");
+  if (!fp) fp = stdin;
+  char line[256], type[40] = {0}, sampling[40] = {0}, cfgdir[200] = {0};
+  float T0 = 0, Tf = 0, t0 = 0, tf = 0, period = 1, gr_range = 0;
+  double dt = 0, trans_time = 0, run_time = 0, minimal_decade = 0, force_range = 0;
+  int ngap_rescaling = 0, nsamples = 0, tagb = 0, tags = 0, n_configs = 0, gr_nbins = 0,
+      force_nbins = 0, idum = 0;
+  std::printf("temp_protocol ?\n");
+  read_line(fp, line, sizeof line);
+  std::sscanf(line, "%39s", type);
+  const bool sine = std::strcmp(type, "sine") == 0;
+  read_line(fp, line, sizeof line);
+  if (std::strcmp(type, "linear") == 0)
+    std::sscanf(line, "%f  %f  %f  %f  %d", &T0, &Tf, &t0, &tf, &ngap_rescaling);
+  else if (sine)
+    std::sscanf(line, "%f  %f  %f  %f  %f  %d", &T0, &Tf, &t0, &tf, &period, &ngap_rescaling);
+  else
+    die("unknown temperature protocol: " + std::string(type));
+  read_line(fp, line, sizeof line);
+  std::sscanf(line, "%lf", &dt);
+  read_line(fp, line, sizeof line);
+  std::sscanf(line, "%39s", sampling);
+  const bool log_mode = std::strcmp(sampling, "log") == 0;
+  read_line(fp, line, sizeof line);
+  if (log_mode)
+    std::sscanf(line, "%lf  %lf %lf", &trans_time, &minimal_decade, &run_time);
+  else
+    std::sscanf(line, "%lf %lf %d", &trans_time, &run_time, &nsamples);
+  read_line(fp, line, sizeof line);
+  std::sscanf(line, "%d %d", &tagb, &tags);
+  read_line(fp, line, sizeof line);
+  std::sscanf(line, "%199s", cfgdir);  // "../configs/" (not used, as in the reference)
+  read_line(fp, line, sizeof line);
+  std::sscanf(line, "%d", &n_configs);
+  read_line(fp, line, sizeof line);
+  std::sscanf(line, "%d %f", &gr_nbins, &gr_range);
+  read_line(fp, line, sizeof line);
+  std::sscanf(line, "%d %lf", &force_nbins, &force_range);
+  read_line(fp, line, sizeof line);
+  std::sscanf(line, "%d", &idum);
+  if (fp != stdin) std::fclose(fp);
+  if (n_configs < 1 || gr_nbins < 1 || force_nbins < 1 || !(force_range > 0.0))
+    die("bad run parameters");
+  const int niter = (int)(run_time / dt), ntrans = (int)(trans_time / dt);
+  std::vector<int> sample_at;
+  if (log_mode) {
+    sample_at = sampling_indexes((float)minimal_decade, (float)run_time, (float)dt, 10);
+  } else {
+    if (nsamples < 1) die("nsamples must be >= 1");
+    const int ngap = niter / nsamples;
+    if (ngap < 1) die("error: ngap " + std::to_string(ngap));
+    for (int ni = 0; ni < niter; ni += ngap) sample_at.push_back(ni);
+  }
+  nsamples = (int)sample_at.size();
+  std::vector<InitConfig> cfg(n_configs);
+  for (int i = 0; i < n_configs; i++) {
+    std::string path = "../configs/init_config_" + std::to_string(i + 1);
+    struct stat st;
+    if (stat(path.c_str(), &st) != 0)
+      path = "../configs/decade0/init_config_" + std::to_string(i + 1);
+    cfg[i] = read_init_config(path);
+  }
+  std::vector<bd_params> params(n_configs);
+  for (int i = 0; i < n_configs; i++) {
+    bd_params& p = params[i];
+    std::memset(&p, 0, sizeof p);
+    p.n_big = cfg[i].n_big;
+    p.n_sml = cfg[i].n_sml;
+    p.potential = potential_from_env();
+    p.ntags_big = tagb;
+    p.ntags_sml = tags;
+    p.temp_protocol = sine ? BD_TEMP_SINE : BD_TEMP_LINEAR;
+    p.side = cfg[i].side;
+    p.sigma_big = cfg[i].sigma_big;
+    p.sigma_sml = cfg[i].sigma_sml;
+    p.dt = dt;
+    p.T0 = T0;
+    p.Tf = Tf;
+    p.t0 = t0;
+    p.tf = tf;
+    p.period = period;
+    p.seed = bd_lane_seed((uint64_t)(int64_t)idum, (uint64_t)i);
+    p.first_step = -ntrans;
+  }
+  bd_sys* s = nullptr;
+  check(bd_create(params.data(), n_configs, &s), "bd_create");
+  for (int i = 0; i < n_configs; i++) check(bd_set_positions(s, i, cfg[i].xyz.data()), "set");
+  bd_cv3d* cv = nullptr;
+  check(bd_cv3d_create(force_nbins, force_range, &cv), "bd_cv3d_create");
+  std::vector<double> gders((size_t)nsamples * 4 * gr_nbins, 0.0), e_b(nsamples, 0.0),
+      e_s(nsamples, 0.0), e_T(nsamples, 0.0), press(nsamples, 0.0), t_s(nsamples, 0.0);
+  std::vector<uint64_t> counts((size_t)4 * gr_nbins);
+  if (ntrans > 0) check(bd_step(s, ntrans), "bd_step");
+  check(bd_mark_origin(s), "bd_mark_origin");
+  int64_t ni_next = 0;
+  for (int k = 0; k < nsamples; k++) {
+    const int target = sample_at[k];
+    const int64_t need = (int64_t)target + 1 - ni_next;
+    if (need > 0) check(bd_step(s, (int32_t)need), "bd_step");
+    ni_next += need > 0 ? need : 0;
+    for (int r = 0; r < n_configs; r++) {
+      bd_obs o;
+      check(bd_observables(s, r, 1.0, &o), "bd_observables");
+      e_b[k] += o.epot_big;
+      e_s[k] += o.epot_sml;
+      e_T[k] = o.temperature;
+      press[k] += o.z;
+      std::fill(counts.begin(), counts.end(), 0);
+      check(bd_gder(s, r, gr_nbins, gr_range, counts.data()), "bd_gder");
+      for (size_t q = 0; q < counts.size(); q++)
+        gders[(size_t)k * 4 * gr_nbins + q] += (double)counts[q];
+      check(bd_cv3d_add(cv, s, r, 1), "bd_cv3d_add");
+    }
+    t_s[k] = dt * (1.0 + (double)target);
+    std::printf("run ni %d/%d  --  print %d/%d -- temp %.3f\n", target, niter, k + 1, nsamples,
+                e_T[k]);
+    std::fflush(stdout);
+  }
+  bd_destroy(s);
+  check(bd_cv3d_solve(cv), "bd_cv3d_solve");
+  std::vector<double> force((size_t)force_nbins);
+  check(bd_cv3d_get(cv, force.data()), "bd_cv3d_get");
+  bd_cv3d_destroy(cv);
+  std::printf("printing files\n");
+  const std::string dir = "results/";
+  write_gders(dir, gders, nsamples, gr_nbins, gr_range, cfg[0], n_configs);
+  auto open_out = [&](const std::string& name) {
+    FILE* f = std::fopen((dir + name).c_str(), "w");
+    if (!f) die("Verify file path: " + dir + name, -2);
+    return f;
+  };
+  FILE* f = open_out("energies_averaged.out");
+  for (int k = 0; k < nsamples; k++)
+    std::fprintf(f, "%f  %f  %f  %f\n", t_s[k], e_b[k] / n_configs, e_s[k] / n_configs, e_T[k]);
+  std::fclose(f);
+  f = open_out("pressure.out");
+  for (int k = 0; k < nsamples; k++) std::fprintf(f, "%f  %f\n", t_s[k], press[k] / n_configs);
+  std::fclose(f);
+  f = open_out("force_contraction.out");
+  const double fb = force_range / force_nbins;
+  for (int b = 0; b < force_nbins; b++) std::fprintf(f, "%f  %.10e\n", (0.5 + b) * fb, force[b]);
+  std::fclose(f);
+  std::printf("TERMINADO\n");
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc > 1 && std::strcmp(argv[1], "--make-configs") == 0) return make_configs(argc, argv);
+  if (argc > 1 && std::strcmp(argv[1], "--system") == 0) return run_system();
+  // --aging: in log mode with decade dumps, also measure F_s and the MSD from
+  // every dumped waiting time within this one run (results/decadeK/) instead
+  // of a reheat-and-restart run per decade
+  const bool aging = argc > 1 && std::strcmp(argv[1], "--aging") == 0;
 
   // ---------------------------------------------------------------- protocol
   FILE* fp_data = std::fopen("wca_aging_linear.data", "r");
@@ -414,6 +610,15 @@ int main(int argc, char** argv) {
       t_s(nsamples, 0.0);
   std::vector<uint64_t> counts((size_t)4 * nbins_gder);
   int number_of_tw = 1;
+  // one-run aging: waiting-time origins (slot j = decade j + 1)
+  const int n_origins = aging && log_mode && print_decades_configs
+                            ? std::min(number_of_tws, (int)BD_MAX_ORIGINS)
+                            : 0;
+  int origins_set = 0;
+  std::vector<double> t_origin(n_origins, 0.0);
+  std::vector<std::vector<double>> o_lag(n_origins), o_fs(n_origins), o_msdb(n_origins),
+      o_msds(n_origins);
+  std::vector<double> o_out;
 
   // transient, then ni == 0 resets the aging clock (main.cu:497-506)
   if (ntrans > 0) check(bd_step(s, ntrans), "bd_step");
@@ -465,9 +670,34 @@ int main(int argc, char** argv) {
         write_init_config(path, cfg[r], pos.data());
       }
     }
+    // one-run aging: F_s / MSD against every origin set so far, then a new
+    // origin where this sample dumps a decade configuration
+    if (origins_set > 0) {
+      o_out.assign((size_t)n_configs * origins_set * 3, 0.0);
+      check(bd_fself_origins(s, qmax, origins_set, o_out.data()), "bd_fself_origins");
+      for (int j = 0; j < origins_set; j++) {
+        double fs = 0, mb = 0, ms = 0;
+        for (int r = 0; r < n_configs; r++) {
+          const double* o = &o_out[((size_t)r * origins_set + j) * 3];
+          fs += o[0];
+          mb += o[1];
+          ms += o[2];
+        }
+        o_lag[j].push_back(time - t_origin[j]);
+        o_fs[j].push_back(fs / n_configs);
+        o_msdb[j].push_back(mb / n_configs);
+        o_msds[j].push_back(ms / n_configs);
+      }
+    }
     if (log_mode && k % points_per_decade == 1 && print_decades_configs &&
-        number_of_tw <= number_of_tws)
+        number_of_tw <= number_of_tws) {
+      if (number_of_tw - 1 < n_origins) {
+        check(bd_set_origin(s, number_of_tw - 1), "bd_set_origin");
+        t_origin[number_of_tw - 1] = time;
+        origins_set = number_of_tw;
+      }
       number_of_tw++;
+    }
     t_s[k] = time;
     std::printf("run ni %d/%d  %.2f%%  --  print %d/%d  --  ocup b s  %d %d -- temp %.3f\n", target,
                 niter, 100.0 * (double)target / (double)niter, k + 1, nsamples, ocb, ocs, e_T[k]);
@@ -477,22 +707,6 @@ int main(int argc, char** argv) {
   for (int i = 0; i < n_configs; i++) std::printf("Finished step %d of %d\n", i + 1, n_configs);
 
   // ---------------------------------------------------------------- normalise and write
-  const double bin = (double)range_gder / (double)nbins_gder;
-  const double volume = (double)c0.side * c0.side * c0.side;
-  const double xb = c0.n_big, xs = c0.n_sml;
-  const double sbs = 0.5 * ((double)c0.sigma_big + (double)c0.sigma_sml);
-  auto vfree = [&](double sg) { return volume - (4.0 / 3.0) * kPiRef * sg * sg * sg; };
-  for (int k = 0; k < nsamples; k++)
-    for (int nb = 0; nb < nbins_gder; nb++) {
-      const double xnb = nb;
-      const double shell = 4.0 * kPiRef * bin * bin * bin * ((1.0 / 3.0) + xnb * xnb + xnb);
-      double* gk = &gders[(size_t)k * 4 * nbins_gder];
-      gk[nb] = gk[nb] / shell / (xb * (xb - 1.0) / vfree(c0.sigma_big)) / n_configs;
-      gk[nbins_gder + nb] = gk[nbins_gder + nb] / shell / (xb * xs / vfree(sbs)) / n_configs;
-      gk[2 * nbins_gder + nb] = gk[2 * nbins_gder + nb] / shell / (xs * xb / vfree(sbs)) / n_configs;
-      gk[3 * nbins_gder + nb] =
-          gk[3 * nbins_gder + nb] / shell / (xs * (xs - 1.0) / vfree(c0.sigma_sml)) / n_configs;
-    }
   std::printf("printing files\n");
   const std::string dir = log_mode ? "results/decade" + std::to_string(current_decade) + "/"
                                    : std::string("results/");
@@ -505,14 +719,7 @@ int main(int argc, char** argv) {
     }
     return f;
   };
-  for (int k = 0; k < nsamples; k++) {
-    FILE* f = open_out("gder_" + std::to_string(k) + ".out");
-    const double* gk = &gders[(size_t)k * 4 * nbins_gder];
-    for (int nb = 0; nb < nbins_gder; nb++)
-      std::fprintf(f, "%f  %f  %f  %f  %f\n", (0.5 + nb) * bin, gk[nb], gk[nbins_gder + nb],
-                   gk[2 * nbins_gder + nb], gk[3 * nbins_gder + nb]);
-    std::fclose(f);
-  }
+  write_gders(dir, gders, nsamples, nbins_gder, range_gder, c0, n_configs);
   FILE* f = open_out("energies_averaged.out");
   for (int k = 0; k < nsamples; k++)
     std::fprintf(f, "%f  %f  %f  %f\n", t_s[k], e_b[k] / n_configs, e_s[k] / n_configs, e_T[k]);
@@ -527,6 +734,22 @@ int main(int argc, char** argv) {
   f = open_out("pressure.out");
   for (int k = 0; k < nsamples; k++) std::fprintf(f, "%f  %f\n", t_s[k], press[k] / n_configs);
   std::fclose(f);
+  for (int j = 0; j < n_origins; j++) {
+    const std::string od = "results/decade" + std::to_string(j + 1) + "/";
+    FILE* fo = std::fopen((od + "fself.out").c_str(), "w");
+    FILE* fm = std::fopen((od + "msd_averaged.out").c_str(), "w");
+    if (!fo || !fm) {
+      std::printf("Verify file path: %s", od.c_str());
+      std::fflush(stdout);
+      std::exit(-2);
+    }
+    for (size_t k = 0; k < o_lag[j].size(); k++) {
+      std::fprintf(fo, "%f  %f\n", o_lag[j][k], o_fs[j][k]);
+      std::fprintf(fm, "%f  %f  %f\n", o_lag[j][k], o_msdb[j][k], o_msds[j][k]);
+    }
+    std::fclose(fo);
+    std::fclose(fm);
+  }
   if (fp_bitac) std::fclose(fp_bitac);
   std::printf("TERMINADO\n");
   return 0;

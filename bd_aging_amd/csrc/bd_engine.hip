@@ -1827,6 +1827,50 @@ int device_state(bd_sys* s, int replica, DevState* out) {
 // ===========================================================================
 // C ABI
 // ===========================================================================
+namespace {
+struct CkptHeader {
+  char magic[8];
+  uint32_t version;
+  int32_t replicas;
+  int64_t step;
+  uint64_t particles, lanes;
+  uint32_t has_origins, pad;
+};
+
+template <class T>
+int ckpt_dev(FILE* f, T* dev, size_t count, bool save, hipStream_t st) {
+  std::vector<T> h(count);
+  if (save) {
+    HIP_TRY(hipMemcpyAsync(h.data(), dev, sizeof(T) * count, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (std::fwrite(h.data(), sizeof(T), count, f) != count)
+      return fail(BD_E_ARG, "checkpoint write failed");
+  } else {
+    if (std::fread(h.data(), sizeof(T), count, f) != count)
+      return fail(BD_E_ARG, "checkpoint file is truncated");
+    HIP_TRY(hipMemcpyAsync(dev, h.data(), sizeof(T) * count, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  return BD_OK;
+}
+
+int ckpt_arrays(bd_sys* s, FILE* f, bool save) {
+  const size_t N = (size_t)s->n_total;
+  double* arrs[] = {s->px, s->py, s->pz, s->rx, s->ry, s->rz, s->ix,
+                    s->iy, s->iz, s->fx, s->fy, s->fz, s->vir, s->pot};
+  int rc = BD_OK;
+  for (double* a : arrs)
+    if (rc == BD_OK) rc = ckpt_dev(f, a, N, save, s->stream);
+  if (rc == BD_OK)
+    rc = ckpt_dev(f, s->ring, (size_t)bd::kRingLen * s->lanes_total, save, s->stream);
+  if (rc == BD_OK) rc = ckpt_dev(f, s->lcg, (size_t)s->lanes_total, save, s->stream);
+  if (rc == BD_OK) rc = ckpt_dev(f, s->draws, (size_t)s->R, save, s->stream);
+  if (rc == BD_OK && s->origins)
+    rc = ckpt_dev(f, s->origins, (size_t)BD_MAX_ORIGINS * 3 * N, save, s->stream);
+  return rc;
+}
+}  // namespace
+
 extern "C" {
 
 const char* bd_last_error(void) { return g_err.c_str(); }
@@ -2568,6 +2612,74 @@ int bd_get_force_order(const bd_sys* s, int32_t* order) {
 int bd_get_brick(const bd_sys* s, int32_t* brick) {
   if (!s || !brick) return fail(BD_E_ARG, "null argument");
   *brick = s->brick;
+  return BD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Checkpoint / resume.  File layout (native endianness):
+//   "BDCKPT01", u32 version = 1, i32 replicas, i64 step, u64 particles,
+//   u64 lanes, u32 has_origins, u32 0, then bd_params[replicas], then the
+//   FP64 arrays pos x,y,z, raw x,y,z, ini x,y,z, F x,y,z, vir, pot
+//   (particles each), the rannew64 rings (250 x lanes u64), lcg (lanes),
+//   draws (replicas) and, if stored, the waiting-time origins
+//   (BD_MAX_ORIGINS x 3 x particles FP64).
+// ---------------------------------------------------------------------------
+
+int bd_checkpoint_save(bd_sys* s, const char* path) {
+  if (!s || !path) return fail(BD_E_ARG, "null argument");
+  int rc = require_positions(s);
+  if (rc != BD_OK) return rc;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(BD_E_ARG, std::string("cannot open ") + path);
+  CkptHeader h{};
+  std::memcpy(h.magic, "BDCKPT01", 8);
+  h.version = 1;
+  h.replicas = s->R;
+  h.step = s->ni;
+  h.particles = (uint64_t)s->n_total;
+  h.lanes = (uint64_t)s->lanes_total;
+  h.has_origins = s->origins ? 1u : 0u;
+  if (std::fwrite(&h, sizeof(h), 1, f) != 1 ||
+      std::fwrite(s->params.data(), sizeof(bd_params), s->params.size(), f) != s->params.size())
+    rc = fail(BD_E_ARG, "checkpoint write failed");
+  if (rc == BD_OK) rc = ckpt_arrays(s, f, true);
+  if (std::fclose(f) != 0 && rc == BD_OK) rc = fail(BD_E_ARG, "checkpoint close failed");
+  return rc;
+}
+
+int bd_checkpoint_load(bd_sys* s, const char* path) {
+  if (!s || !path) return fail(BD_E_ARG, "null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(BD_E_ARG, std::string("cannot open ") + path);
+  CkptHeader h{};
+  int rc = BD_OK;
+  std::vector<bd_params> ps(s->R);
+  if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, "BDCKPT01", 8) != 0 ||
+      h.version != 1)
+    rc = fail(BD_E_ARG, "not a checkpoint of this engine version");
+  else if (h.replicas != s->R || h.particles != (uint64_t)s->n_total ||
+           h.lanes != (uint64_t)s->lanes_total)
+    rc = fail(BD_E_ARG, "checkpoint holds a different number of replicas, particles or lanes");
+  else if (std::fread(ps.data(), sizeof(bd_params), ps.size(), f) != ps.size() ||
+           std::memcmp(ps.data(), s->params.data(), sizeof(bd_params) * ps.size()) != 0)
+    rc = fail(BD_E_ARG, "checkpoint parameters differ from the engine's");
+  if (rc == BD_OK) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (h.has_origins && !s->origins)
+      rc = dev_alloc(&s->origins, (size_t)BD_MAX_ORIGINS * 3 * s->n_total);
+    if (rc == BD_OK && !h.has_origins && s->origins) {
+      (void)hipFree(s->origins);
+      s->origins = nullptr;
+    }
+  }
+  if (rc == BD_OK) rc = ckpt_arrays(s, f, false);
+  std::fclose(f);
+  if (rc != BD_OK) return rc;
+  s->ni = h.step;
+  HIP_TRY(hipMemcpy(s->step, &s->ni, sizeof(int64_t), hipMemcpyHostToDevice));
+  for (int r = 0; r < s->R; r++) s->have_pos[r] = true;
+  s->located = false;  // cells are rebuilt (and the stored forces recomputed) on the next step
   return BD_OK;
 }
 

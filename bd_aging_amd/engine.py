@@ -223,6 +223,15 @@ class Engine:
         check(self.lib.bd_get_brick(self.h, C.byref(v)))
         return v.value
 
+    def checkpoint_save(self, path):
+        """Write the whole engine state (bit-exact resume) to `path`."""
+        check(self.lib.bd_checkpoint_save(self.h, str(path).encode()))
+
+    def checkpoint_load(self, path):
+        """Restore a state written by checkpoint_save into an engine made with
+        the same parameters."""
+        check(self.lib.bd_checkpoint_load(self.h, str(path).encode()))
+
     def kernel_times(self, nsteps):
         """ms spent in (update, cells, forces) over nsteps eager steps."""
         ms = np.zeros(3)

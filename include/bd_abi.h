@@ -189,6 +189,17 @@ enum { BD_ORDER_REFERENCE = 1, BD_ORDER_SPLIT4 = 4, BD_ORDER_SPLIT8 = 8 };
 int bd_set_force_order(bd_sys* s, int32_t order);
 int bd_get_force_order(const bd_sys* s, int32_t* order);
 
+/* Bit-exact checkpoint of the whole engine state (no reference counterpart;
+ * the reference's decade dumps, main.cu:809-835, keep only %f positions):
+ * positions, raw and initial positions, forces, virials, potentials, the
+ * rannew64 lanes, the step index and any waiting-time origins.  Load into an
+ * engine created with the same parameters: the run continues exactly as the
+ * saved one would have (cells are rebuilt on the next step).  BD_E_ARG for a
+ * file that is unreadable, truncated or made by an engine with other
+ * parameters. */
+int bd_checkpoint_save(bd_sys* s, const char* path);
+int bd_checkpoint_load(bd_sys* s, const char* path);
+
 /* Measurement hook (no reference counterpart): runs nsteps steps with the
  * kernels launched one by one on the engine's stream and HIP events around
  * each; adds the elapsed milliseconds of the update, cells and forces kernels

@@ -36,3 +36,11 @@ def test_unknown_protocol_is_rejected(exe, tmp_path):
     r = subprocess.run([exe], input="cubic\n" + PROTOCOL.split("\n", 1)[1], cwd=tmp_path,
                        capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
+
+
+def test_run_aging_script_parses():
+    import subprocess
+    script = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "bd_aging_amd", "run_aging.sh")
+    r = subprocess.run(["bash", "-n", script], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -92,3 +92,147 @@ def test_driver_matches_oracle_replay(tmp_path):
         g = np.loadtxt(res / f"gder_{k}.out")
         assert g.shape == (nbins, 5)
         assert np.allclose(g[:, 1:], (gsum[k] / 2).T, atol=2e-6, rtol=1e-6)
+
+
+AGING_PROTOCOL = """linear
+1.0  0.1  0.0  0.0  1
+1e-4
+log
+0.002  0.001 0.1
+1  2  0
+8 8
+2
+30 3.0
+7.0
+99
+"""
+
+
+def _sinc_stats(raw, org, nb, q):
+    d = np.sqrt(((raw - org) ** 2).sum(1))
+    qd = q * d[:nb]
+    fs = np.where(qd > 0, np.sin(qd) / np.where(qd > 0, qd, 1.0), 1.0).mean()
+    return fs, (d[:nb] ** 2).mean(), (d[nb:] ** 2).mean()
+
+
+def test_aging_mode_one_run_origins_match_oracle(tmp_path):
+    """--aging: F_s and the MSD from every dumped waiting time in one run."""
+    cfg = tmp_path / "configs"
+    sim = tmp_path / "sim"
+    for k in range(3):
+        (sim / "results" / f"decade{k}").mkdir(parents=True)
+    r = subprocess.run([EXE, "--make-configs", "250", "250", "0.45", "1.4", "1.0", "2", "78",
+                        str(cfg / "decade0"), "1.0", "50", "1e-4"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for k in (1, 2):
+        (cfg / f"decade{k}").mkdir()
+    r = subprocess.run([EXE, "--aging"], input=AGING_PROTOCOL, cwd=sim, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dt, ntrans, q = 1e-4, 20, 7.0
+    samples = orc.sampling_indexes(0.001, 0.1, dt, 10)
+    origins = {}  # slot -> (t_w, [raw per config])
+    got = {0: [], 1: []}
+    systems = []
+    for i in range(2):
+        nb, ns, side, sb, ss, xyz = read_config(cfg / "decade0" / f"init_config_{i + 1}")
+        p = make_params(nb, ns, side=side, sigma_big=sb, sigma_sml=ss, dt=dt,
+                        potential=abi.BD_POT_WCA_MODIFIED, T0=1.0, Tf=0.1, t0=0.0, tf=0.0,
+                        seed=lane_seed(99, i), first_step=-ntrans)
+        o = orc.System(p)
+        o.set_positions(xyz)
+        assert o.cell_locate() == 0
+        o.compute_forces()
+        assert o.step(ntrans) == 0
+        o.mark_origin()
+        systems.append((o, nb))
+    done, n_tw = 0, 1
+    for k, target in enumerate(samples):
+        if target + 1 < done:
+            continue
+        if target + 1 > done:
+            for o, _ in systems:
+                assert o.step(target + 1 - done) == 0
+            done = target + 1
+        time = dt * (1.0 + target)
+        raws = [o.raw_positions() for o, _ in systems]
+        for j, (tw, orgs) in origins.items():
+            st = np.array([_sinc_stats(raws[i], orgs[i], systems[i][1], q) for i in range(2)])
+            got[j].append((time - tw, *st.mean(0)))
+        if k % 10 == 1 and n_tw <= 2:
+            origins[n_tw - 1] = (time, raws)
+            n_tw += 1
+    for j in (0, 1):
+        exp = np.array(got[j])
+        fs = np.loadtxt(sim / "results" / f"decade{j + 1}" / "fself.out", ndmin=2)
+        msd = np.loadtxt(sim / "results" / f"decade{j + 1}" / "msd_averaged.out", ndmin=2)
+        assert fs.shape[0] == exp.shape[0] > 0
+        assert np.allclose(fs[:, 0], exp[:, 0], atol=2e-6)
+        assert np.allclose(fs[:, 1], exp[:, 1], atol=2e-6)
+        assert np.allclose(msd[:, 1:], exp[:, 2:], atol=2e-6)
+    # the decade dumps are still written (text-dump compatibility)
+    assert (cfg / "decade1" / "init_config_1").exists()
+    assert (cfg / "decade2" / "init_config_2").exists()
+
+
+SYSTEM_PROTOCOL = """linear
+1.0  1.0  0.0  0.0  1
+1e-4
+linear
+0.002  0.004  2
+8 8
+../configs/
+2
+30 3.0
+24 2.4
+321
+"""
+
+
+def test_system_mode_force_contraction_matches_oracle(tmp_path):
+    """--system (run-system.sh protocol): the contraction over every sample."""
+    cfg = tmp_path / "configs"
+    sim = tmp_path / "sim"
+    (sim / "results").mkdir(parents=True)
+    r = subprocess.run([EXE, "--make-configs", "200", "200", "0.42", "1.4", "1.0", "2", "79",
+                        str(cfg), "1.0", "30", "1e-4"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = subprocess.run([EXE, "--system"], input=SYSTEM_PROTOCOL, cwd=sim, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dt, ntrans, niter, nsamples, bins, rng = 1e-4, 20, 40, 2, 24, 2.4
+    samples = list(range(0, niter, niter // nsamples))
+    ata = np.zeros((3, bins, bins))
+    atb = np.zeros((3, bins))
+    systems = []
+    for i in range(2):
+        nb, ns, side, sb, ss, xyz = read_config(cfg / f"init_config_{i + 1}")
+        p = make_params(nb, ns, side=side, sigma_big=sb, sigma_sml=ss, dt=dt,
+                        potential=abi.BD_POT_WCA_MODIFIED, T0=1.0, Tf=1.0, t0=0.0, tf=0.0,
+                        seed=lane_seed(321, i), first_step=-ntrans)
+        o = orc.System(p)
+        o.set_positions(xyz)
+        assert o.cell_locate() == 0
+        o.compute_forces()
+        assert o.step(ntrans) == 0
+        o.mark_origin()
+        systems.append((o, side))
+    done = 0
+    for target in samples:
+        for o, side in systems:
+            assert o.step(target + 1 - done) == 0
+        done = target + 1
+        for o, side in systems:
+            a, b = orc.cv3d_accumulate(bins, rng, o.positions(), o.forces()[0], side)
+            ata += a
+            atb += b
+    exp = orc.cv3d_solve(ata, atb)
+    got = np.loadtxt(sim / "results" / "force_contraction.out")
+    assert got.shape == (bins, 2)
+    assert np.allclose(got[:, 0], (np.arange(bins) + 0.5) * rng / bins, atol=2e-6)
+    assert np.allclose(got[:, 1], exp, rtol=1e-6, atol=1e-8 * np.abs(exp).max())
+    en = np.loadtxt(sim / "results" / "energies_averaged.out")
+    assert en.shape == (nsamples, 4)
+    assert (sim / "results" / "gder_1.out").exists()

@@ -442,3 +442,41 @@ def test_autotune_keeps_results_and_caches(config1_positions, tmp_path):
     assert e2.autotune(5, cache) == b  # served from the cache
     e.close()
     e2.close()
+
+
+# ------------------------------------------------------------------ checkpoint
+def test_checkpoint_resume_bit_exact(config1_positions, tmp_path):
+    side, x = config1_positions
+    specs = [(1000, 1000, side, 0.3, 21), (1000, 1000, side, 0.1, 22)]
+    params = [make_params(nb, ns, side=sd, T0=1.0, Tf=T, first_step=-10, seed=sd_)
+              for nb, ns, sd, T, sd_ in specs]
+    a = Engine(params)
+    for r in range(2):
+        a.set_positions(x, r)
+    a.step(30)
+    a.set_origin(0)
+    a.step(20)
+    ck = str(tmp_path / "state.bdck")
+    a.checkpoint_save(ck)
+    a.step(30)
+    b = Engine(params)
+    b.checkpoint_load(ck)
+    assert b.step_index() == a.step_index() - 30
+    b.step(30)
+    for r in range(2):
+        assert np.array_equal(a.positions(r), b.positions(r))
+        assert np.array_equal(a.raw_positions(r), b.raw_positions(r))
+        assert all(np.array_equal(u, v) for u, v in zip(a.forces(r), b.forces(r)))
+    assert np.array_equal(a.fself_origins(7.0, 1), b.fself_origins(7.0, 1))
+    assert a.step_index() == b.step_index()
+    # another engine's parameters, and a truncated file, are refused
+    c = Engine([make_params(1000, 1000, side=side, seed=5)] * 2)
+    with pytest.raises(BDError):
+        c.checkpoint_load(ck)
+    raw = open(ck, "rb").read()
+    bad = str(tmp_path / "short.bdck")
+    open(bad, "wb").write(raw[: len(raw) // 2])
+    with pytest.raises(BDError):
+        b.checkpoint_load(bad)
+    for e in (a, b, c):
+        e.close()
