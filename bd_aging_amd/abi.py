@@ -92,6 +92,8 @@ SIGNATURES = {
     "bd_autotune": (C.c_int, [_P, C.c_int32, C.c_char_p, _I]),
     "bd_set_force_order": (C.c_int, [_P, C.c_int32]),
     "bd_get_force_order": (C.c_int, [_P, _I]),
+    "bd_set_persistent": (C.c_int, [_P, C.c_int32]),
+    "bd_get_persistent": (C.c_int, [_P, _I]),
     "bd_set_graph_steps": (C.c_int, [_P, C.c_int32]),
     "bd_get_graph_steps": (C.c_int, [_P, _I]),
     "bd_set_walk": (C.c_int, [_P, C.c_int32]),

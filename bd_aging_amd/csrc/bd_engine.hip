@@ -36,6 +36,7 @@
 //   k_forces      per-particle WCA forces over the same-species then the
 //                 cross-species stencil in the reference's traversal order
 //                 (get_forces.cu:297-558), written (not accumulated)
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -204,8 +205,10 @@ __global__ void __launch_bounds__(kBlk)
 // One thread per lane; lane l serves particles l, l+L, l+2L ... and takes
 // four draws per served slot, so every lane advances 4*ceil(N/L) draws per
 // step and the ring pointers np, nq are common to the replica.
-__global__ void __launch_bounds__(kBlk)
-    k_update_bin(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+// One workgroup's share (block-table entry vb) of the update; the kernel
+// below runs it for blockIdx.x, the persistent step kernel for every entry.
+__device__ __forceinline__ void k_update_bin_block(
+    int vb, const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
                  double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
                  double* __restrict__ rx, double* __restrict__ ry, double* __restrict__ rz,
                  const double* __restrict__ fx, const double* __restrict__ fy,
@@ -213,7 +216,7 @@ __global__ void __launch_bounds__(kBlk)
                  uint64_t* __restrict__ lcgs, const uint64_t* __restrict__ draws,
                  const int64_t* __restrict__ step, int total_lanes, int* __restrict__ nocup,
                  int* __restrict__ ins, int* __restrict__ flags, int do_bin) {
-  const int4 t = tbl[blockIdx.x];
+  const int4 t = tbl[vb];
   if (t.x < 0) return;
   const int l = t.y + (int)threadIdx.x;
   if (l >= t.z) return;
@@ -265,6 +268,19 @@ __global__ void __launch_bounds__(kBlk)
   if (bad) atomicOr(&flags[t.x], kFlagNonFinite);
 }
 
+__global__ void __launch_bounds__(kBlk)
+    k_update_bin(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+                 double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+                 double* __restrict__ rx, double* __restrict__ ry, double* __restrict__ rz,
+                 const double* __restrict__ fx, const double* __restrict__ fy,
+                 const double* __restrict__ fz, uint64_t* __restrict__ ring,
+                 uint64_t* __restrict__ lcgs, const uint64_t* __restrict__ draws,
+                 const int64_t* __restrict__ step, int total_lanes, int* __restrict__ nocup,
+                 int* __restrict__ ins, int* __restrict__ flags, int do_bin) {
+  k_update_bin_block((int)blockIdx.x, reps, tbl, px, py, pz, rx, ry, rz, fx, fy, fz, ring, lcgs,
+                     draws, step, total_lanes, nocup, ins, flags, do_bin);
+}
+
 // Row packing.  A wavefront packs rpw = 64 / n rows (x, y) of a grid with n
 // <= 64 cells per row (lane j n + z holds cell z of its j-th row), or one
 // row in chunks of 64 cells when n > 64.  Each lane reads its cell's
@@ -278,18 +294,18 @@ constexpr int kRowWaves = kBlk / 64;
 
 __host__ __device__ inline int rows_per_wave(int n) { return n <= 64 ? 64 / n : 1; }
 
-__global__ void __launch_bounds__(kBlk)
-    k_cells(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+__device__ __forceinline__ void k_cells_block(
+    int vb, const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
             const double* __restrict__ px, const double* __restrict__ py,
             const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
             const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
             int* __restrict__ flags, int64_t* __restrict__ step,
             uint64_t* __restrict__ draws, int n_reps, int advance) {
-  if (advance && blockIdx.x == 0 && threadIdx.x == 0) {
+  if (advance && vb == 0 && threadIdx.x == 0) {
     *step = *step + 1;
     for (int r = 0; r < n_reps; r++) draws[r] += 4ULL * (uint64_t)reps[r].per_lane;
   }
-  const int4 t = tbl[blockIdx.x];
+  const int4 t = tbl[vb];
   if (t.x < 0) return;
   const int g = t.w;
   const RepDev& R = reps[t.x];
@@ -366,6 +382,17 @@ __global__ void __launch_bounds__(kBlk)
   }
   if (row_live && zl == 0) rrow[n] = row_base + carry;
   if (over) atomicOr(&flags[t.x], kFlagOverflow);
+}
+
+__global__ void __launch_bounds__(kBlk)
+    k_cells(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+            const double* __restrict__ px, const double* __restrict__ py,
+            const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
+            const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
+            int* __restrict__ flags, int64_t* __restrict__ step,
+            uint64_t* __restrict__ draws, int n_reps, int advance) {
+  k_cells_block((int)blockIdx.x, reps, tbl, px, py, pz, nocup, cnt, ins, rs, cpos, flags, step,
+                draws, n_reps, advance);
 }
 
 // Distance along one axis from a point to the cell at offset d of its home
@@ -983,8 +1010,8 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
 // DEPTH: per-thread LDS entries for deferred pair terms (0 = direct);
 // WALK: 0 row walk, 1 slab walk.
 template <int POT, bool NEIGH, int DEPTH, int WALK>
-__global__ void __launch_bounds__(kBlk)
-    k_forces(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+__device__ __forceinline__ void k_forces_block(
+    int vb, const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
              const double* __restrict__ px, const double* __restrict__ py,
              const double* __restrict__ pz, const int* __restrict__ rs,
              const double4* __restrict__ cpos, double* __restrict__ ffx,
@@ -993,7 +1020,7 @@ __global__ void __launch_bounds__(kBlk)
              int maxn) {
   __shared__ unsigned int dbuf[DEPTH > 0 ? DEPTH * kBlk : 1];
   __shared__ unsigned int sbuf[WALK == 1 ? 2 * kSlabSegs * kBlk : 1];
-  const int4 t = tbl[blockIdx.x];
+  const int4 t = tbl[vb];
   if (t.x < 0) return;
   const int i = t.y + (int)threadIdx.x;
   if (i >= t.z) return;
@@ -1019,6 +1046,58 @@ __global__ void __launch_bounds__(kBlk)
   ffz[p] = fz;
   vir[p] = v;
   pot[p] = u;
+}
+
+template <int POT, bool NEIGH, int DEPTH, int WALK>
+__global__ void __launch_bounds__(kBlk)
+    k_forces(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+             const double* __restrict__ px, const double* __restrict__ py,
+             const double* __restrict__ pz, const int* __restrict__ rs,
+             const double4* __restrict__ cpos, double* __restrict__ ffx,
+             double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
+             double* __restrict__ pot, int* __restrict__ ncounts, int* __restrict__ nlist,
+             int maxn) {
+  k_forces_block<POT, NEIGH, DEPTH, WALK>((int)blockIdx.x, reps, tbl, px, py, pz, rs, cpos, ffx,
+                                          ffy, ffz, vir, pot, ncounts, nlist, maxn);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent step kernel: nsteps whole steps (update -> cells -> forces) in
+// one cooperative launch.  Every workgroup walks the three block tables with
+// stride gridDim.x, and grid-wide barriers separate the phases, so the state
+// after each barrier is exactly the state after the corresponding separate
+// kernel.  The forces use the slab walk with deferral depth 16 (every
+// force-kernel shape gives bit-identical results).
+// ---------------------------------------------------------------------------
+struct StepArgs {
+  const RepDev* reps;
+  const int4 *tbl_lane, *tbl_cell, *tbl_part;
+  int nb_lane, nb_cell, nb_part, total_lanes, n_reps;
+  double *px, *py, *pz, *rx, *ry, *rz, *fx, *fy, *fz, *vir, *pot;
+  uint64_t *ring, *lcg, *draws;
+  int64_t* step;
+  int *nocup, *cnt, *ins, *rs, *flags;
+  double4* cpos;
+};
+
+template <int POT>
+__global__ void __launch_bounds__(kBlk) k_persistent_steps(StepArgs a, int nsteps) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  for (int st = 0; st < nsteps; st++) {
+    for (int vb = (int)blockIdx.x; vb < a.nb_lane; vb += (int)gridDim.x)
+      k_update_bin_block(vb, a.reps, a.tbl_lane, a.px, a.py, a.pz, a.rx, a.ry, a.rz, a.fx, a.fy,
+                         a.fz, a.ring, a.lcg, a.draws, a.step, a.total_lanes, a.nocup, a.ins,
+                         a.flags, 1);
+    grid.sync();
+    for (int vb = (int)blockIdx.x; vb < a.nb_cell; vb += (int)gridDim.x)
+      k_cells_block(vb, a.reps, a.tbl_cell, a.px, a.py, a.pz, a.nocup, a.cnt, a.ins, a.rs,
+                    a.cpos, a.flags, a.step, a.draws, a.n_reps, 1);
+    grid.sync();
+    for (int vb = (int)blockIdx.x; vb < a.nb_part; vb += (int)gridDim.x)
+      k_forces_block<POT, false, 16, 1>(vb, a.reps, a.tbl_part, a.px, a.py, a.pz, a.rs, a.cpos,
+                                        a.fx, a.fy, a.fz, a.vir, a.pot, nullptr, nullptr, 0);
+    grid.sync();
+  }
 }
 
 // FAST force order: G consecutive lanes share one particle, each walking
@@ -1562,6 +1641,8 @@ struct bd_sys {
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;       // steps in the instantiated graph
   int graph_mode = 16;       // steps per graph launch; 0 = eager
+  int persist = 0;           // steps per persistent launch; 0 = off
+  int persist_grid = 0;      // co-resident workgroups of the persistent kernel
   // brick force kernel (0 = thread-per-particle global kernel)
   int brick = 0;
   std::vector<int4> tbl_brick_h;
@@ -1701,6 +1782,41 @@ void launch_forces(bd_sys* s) {
     launch_brick_t<2048>(s);
   }
 }
+
+template <int POT>
+int launch_persistent_t(bd_sys* s, int nsteps) {
+  const void* fn = reinterpret_cast<const void*>(&k_persistent_steps<POT>);
+  if (s->persist_grid == 0) {
+    int dev = 0, coop = 0, ncu = 0, per_cu = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    if (!coop) return fail(BD_E_STATE, "the device does not support cooperative launches");
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlk, 0));
+    if (per_cu < 1) return fail(BD_E_STATE, "the persistent kernel does not fit on a CU");
+    s->persist_grid = per_cu * ncu;
+  }
+  StepArgs a{s->reps_d, s->tbl_lane, s->tbl_cell, s->tbl_part,
+             (int)s->tbl_lane_h.size(), (int)s->tbl_cell_h.size(), (int)s->tbl_part_h.size(),
+             s->lanes_total, s->R, s->px, s->py, s->pz, s->rx, s->ry, s->rz, s->fx, s->fy,
+             s->fz, s->vir, s->pot, s->ring, s->lcg, s->draws, s->step, s->nocup, s->cnt,
+             s->ins, s->rs, s->flags, s->cpos};
+  void* args[] = {&a, &nsteps};
+  HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(s->persist_grid), dim3(kBlk), args, 0, s->stream));
+  return BD_OK;
+}
+
+int launch_persistent(bd_sys* s, int nsteps) {
+  switch (s->pot_kind) {
+    case BD_POT_WCA:
+      return launch_persistent_t<BD_POT_WCA>(s, nsteps);
+    case BD_POT_WCA_MODIFIED:
+      return launch_persistent_t<BD_POT_WCA_MODIFIED>(s, nsteps);
+    default:
+      return launch_persistent_t<BD_POT_HARMONIC>(s, nsteps);
+  }
+}
+
 
 // Largest number of cells a brick of B big cells stages (both boxes).
 int brick_cells_bound(const RepDev& R, int B) {
@@ -2384,6 +2500,19 @@ int bd_step(bd_sys* s, int32_t nsteps) {
     launch_forces(s);
     s->located = true;
   }
+  if (s->persist > 0) {
+    if (s->order != BD_ORDER_REFERENCE)
+      return fail(BD_E_STATE, "the persistent step kernel computes forces in reference order");
+    for (int left = nsteps; left > 0;) {
+      const int k = left < s->persist ? left : s->persist;
+      rc = launch_persistent(s, k);
+      if (rc != BD_OK) return rc;
+      left -= k;
+    }
+    HIP_TRY(hipGetLastError());
+    s->ni += nsteps;
+    return check_flags(s);
+  }
   const int G = s->graph_mode;
   int left = nsteps;
   if (G > 0 && left >= G) {
@@ -2412,6 +2541,22 @@ int bd_set_graph_steps(bd_sys* s, int32_t steps) {
     s->graph_steps = 0;
   }
   s->graph_mode = steps;
+  return BD_OK;
+}
+
+int bd_set_persistent(bd_sys* s, int32_t steps) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (steps < 0 || steps > 100000) return fail(BD_E_ARG, "steps must be 0..100000");
+  if (steps > 0 && !s->defer_ok)
+    return fail(BD_E_ARG, "a grid has too many slots for the persistent kernel's force walk");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->persist = steps;
+  return BD_OK;
+}
+
+int bd_get_persistent(const bd_sys* s, int32_t* steps) {
+  if (!s || !steps) return fail(BD_E_ARG, "null argument");
+  *steps = s->persist;
   return BD_OK;
 }
 

@@ -188,6 +188,15 @@ class Engine:
         check(self.lib.bd_get_force_order(self.h, C.byref(v)))
         return v.value
 
+    def set_persistent(self, steps):
+        """Steps per persistent cooperative launch in step(); 0 switches it off."""
+        check(self.lib.bd_set_persistent(self.h, int(steps)))
+
+    def persistent(self):
+        v = C.c_int32()
+        check(self.lib.bd_get_persistent(self.h, C.byref(v)))
+        return v.value
+
     def set_graph_steps(self, steps):
         """Steps per captured graph in step(); 0 enqueues every kernel eagerly."""
         check(self.lib.bd_set_graph_steps(self.h, int(steps)))

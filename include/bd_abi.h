@@ -143,6 +143,15 @@ int bd_step(bd_sys* s, int32_t nsteps);
 int bd_set_graph_steps(bd_sys* s, int32_t steps);
 int bd_get_graph_steps(const bd_sys* s, int32_t* steps);
 
+/* Persistent launch mode of bd_step (no reference counterpart): steps > 0
+ * runs up to `steps` whole steps per cooperative launch of one kernel whose
+ * co-resident workgroups sweep the update, binning and force phases with
+ * grid-wide barriers between them; 0 (default) returns to graph / eager
+ * launches.  Bit-identical results; reference force order only (BD_E_STATE
+ * from bd_step otherwise). */
+int bd_set_persistent(bd_sys* s, int32_t steps);
+int bd_get_persistent(const bd_sys* s, int32_t* steps);
+
 /* Force-kernel shape (no reference counterpart; the autotuner's knobs):
  * brick = 1..4 selects the LDS-staged kernel over brick^3 big-grid cells,
  * 0 the thread-per-particle row walk.  Every choice gives bit-identical

@@ -480,3 +480,32 @@ def test_checkpoint_resume_bit_exact(config1_positions, tmp_path):
         b.checkpoint_load(bad)
     for e in (a, b, c):
         e.close()
+
+
+def test_persistent_steps_bit_identical(config1_positions):
+    """One cooperative launch per 16 steps (grid barriers between the phases)
+    reproduces the graph-replayed trajectory, also for a batch of replicas."""
+    side, x = config1_positions
+    params = [make_params(1000, 1000, side=side, T0=1.0, Tf=0.2, first_step=-5, seed=31),
+              make_params(1000, 1000, side=side, T0=0.5, Tf=0.5, first_step=-5, seed=32)]
+    out = []
+    for persist in (0, 16):
+        e = Engine(params)
+        e.set_persistent(persist)
+        assert e.persistent() == persist
+        for r in range(2):
+            e.set_positions(x, r)
+        e.step(37)
+        e.set_persistent(0)
+        e.step(5)
+        out.append(([e.positions(r) for r in range(2)], [e.forces(r)[0] for r in range(2)],
+                    e.step_index()))
+        e.close()
+    for r in range(2):
+        assert np.array_equal(out[0][0][r], out[1][0][r])
+        assert np.array_equal(out[0][1][r], out[1][1][r])
+    assert out[0][2] == out[1][2]
+    e = Engine(params)
+    with pytest.raises(BDError):
+        e.set_persistent(-1)
+    e.close()
