@@ -12,7 +12,8 @@
 //   nocup            int per cell   atomic insertion counters (zero between steps)
 //   cnt              int per cell   occupancy of the last binning
 //   ins              int per slot   insertion-order particle ids (ntags slots per
-//                                   cell, cell index x + n (y + n z))
+//                                   cell; cells z-fastest, index z + n (x + n y),
+//                                   for nocup and cnt too)
 //   cpos             double4 per slot  x, y, z, replica-local index, in ROW
 //                    layout: row (x, y) owns the n * ntags slots from
 //                    (x + n y) n ntags; its particles are packed from the
@@ -157,7 +158,7 @@ __device__ inline void insert_cell(const RepDev& R, int rep, int i, double x, do
     return;
   }
   const int n = R.ncell[g];
-  const int c = ii + n * (jj + n * kk);
+  const int c = kk + n * (ii + n * jj);  // z fastest: a row's cells are adjacent
   const int slot = atomicAdd(&nocup[R.cell_off[g] + c], 1);
   if (slot < R.ntags[g]) ins[(size_t)R.tag_off[g] + (size_t)c * R.ntags[g] + slot] = loc;
 }
@@ -288,8 +289,9 @@ __global__ void __launch_bounds__(kBlk)
 // after the other from the row's first slot, and the lane writes its cell's
 // particles ranked by id (ascending, the serial order of cell_locate_hst,
 // cell_locate.cu:72-101) with their positions.  The insertion counters are
-// reset for the next binning.  With advance != 0 the first thread also
-// advances the step index and the per-replica draw counters.
+// reset for the next binning.  With advance != 0 the first workgroup also
+// advances the step index, and the workgroup holding a replica's first
+// big-grid rows that replica's draw counter.
 constexpr int kRowWaves = kBlk / 64;
 
 __host__ __device__ inline int rows_per_wave(int n) { return n <= 64 ? 64 / n : 1; }
@@ -300,12 +302,13 @@ __device__ __forceinline__ void k_cells_block(
             const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
             const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
             int* __restrict__ flags, int64_t* __restrict__ step,
-            uint64_t* __restrict__ draws, int n_reps, int advance) {
-  if (advance && vb == 0 && threadIdx.x == 0) {
-    *step = *step + 1;
-    for (int r = 0; r < n_reps; r++) draws[r] += 4ULL * (uint64_t)reps[r].per_lane;
-  }
+            uint64_t* __restrict__ draws, int advance) {
   const int4 t = tbl[vb];
+  if (advance && threadIdx.x == 0) {
+    if (vb == 0) *step = *step + 1;
+    // each replica's draw counter by the workgroup of its first big-grid rows
+    if (t.x >= 0 && t.y == 0 && t.w == 0) draws[t.x] += 4ULL * (uint64_t)reps[t.x].per_lane;
+  }
   if (t.x < 0) return;
   const int g = t.w;
   const RepDev& R = reps[t.x];
@@ -331,7 +334,7 @@ __device__ __forceinline__ void k_cells_block(
   for (int z0 = 0; z0 < n; z0 += 64) {
     const int z = z0 + zl;
     const bool live = row_live && z < n;
-    const int c = row + n * n * z;  // x + n (y + n z)
+    const int c = row * n + z;  // z + n (x + n y): coalesced across the lanes
     int occ = 0;
     if (live) occ = nocup[R.cell_off[g] + c];
     const int m = occ < ntags ? occ : ntags;
@@ -390,9 +393,9 @@ __global__ void __launch_bounds__(kBlk)
             const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
             const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
             int* __restrict__ flags, int64_t* __restrict__ step,
-            uint64_t* __restrict__ draws, int n_reps, int advance) {
+            uint64_t* __restrict__ draws, int advance) {
   k_cells_block((int)blockIdx.x, reps, tbl, px, py, pz, nocup, cnt, ins, rs, cpos, flags, step,
-                draws, n_reps, advance);
+                draws, advance);
 }
 
 // Distance along one axis from a point to the cell at offset d of its home
@@ -1072,7 +1075,7 @@ __global__ void __launch_bounds__(kBlk)
 struct StepArgs {
   const RepDev* reps;
   const int4 *tbl_lane, *tbl_cell, *tbl_part;
-  int nb_lane, nb_cell, nb_part, total_lanes, n_reps;
+  int nb_lane, nb_cell, nb_part, total_lanes;
   double *px, *py, *pz, *rx, *ry, *rz, *fx, *fy, *fz, *vir, *pot;
   uint64_t *ring, *lcg, *draws;
   int64_t* step;
@@ -1091,7 +1094,7 @@ __global__ void __launch_bounds__(kBlk) k_persistent_steps(StepArgs a, int nstep
     grid.sync();
     for (int vb = (int)blockIdx.x; vb < a.nb_cell; vb += (int)gridDim.x)
       k_cells_block(vb, a.reps, a.tbl_cell, a.px, a.py, a.pz, a.nocup, a.cnt, a.ins, a.rs,
-                    a.cpos, a.flags, a.step, a.draws, a.n_reps, 1);
+                    a.cpos, a.flags, a.step, a.draws, 1);
     grid.sync();
     for (int vb = (int)blockIdx.x; vb < a.nb_part; vb += (int)gridDim.x)
       k_forces_block<POT, false, 16, 1>(vb, a.reps, a.tbl_part, a.px, a.py, a.pz, a.rs, a.cpos,
@@ -1681,7 +1684,7 @@ void launch_cells(bd_sys* s, int advance) {
   const int nb = (int)s->tbl_cell_h.size();
   k_cells<<<nb, kBlk, 0, s->stream>>>(s->reps_d, s->tbl_cell, s->px, s->py, s->pz, s->nocup,
                                       s->cnt, s->ins, s->rs, s->cpos, s->flags, s->step,
-                                      s->draws, s->R, advance);
+                                      s->draws, advance);
 }
 
 template <bool NEIGH, int DEPTH, int WALK>
@@ -1798,7 +1801,7 @@ int launch_persistent_t(bd_sys* s, int nsteps) {
   }
   StepArgs a{s->reps_d, s->tbl_lane, s->tbl_cell, s->tbl_part,
              (int)s->tbl_lane_h.size(), (int)s->tbl_cell_h.size(), (int)s->tbl_part_h.size(),
-             s->lanes_total, s->R, s->px, s->py, s->pz, s->rx, s->ry, s->rz, s->fx, s->fy,
+             s->lanes_total, s->px, s->py, s->pz, s->rx, s->ry, s->rz, s->fx, s->fy,
              s->fz, s->vir, s->pot, s->ring, s->lcg, s->draws, s->step, s->nocup, s->cnt,
              s->ins, s->rs, s->flags, s->cpos};
   void* args[] = {&a, &nsteps};
@@ -2387,21 +2390,24 @@ int bd_get_cells(bd_sys* s, int32_t replica, int32_t species, int32_t* nocup,
   const size_t nc3 = (size_t)n * n * n;
   const int ntags = R.ntags[species];
   const int sp_off = species == 0 ? 0 : R.nb;
-  std::vector<int32_t> rs((size_t)n * n * (n + 1));
+  std::vector<int32_t> rs((size_t)n * n * (n + 1)), occ_int(nc3);
   std::vector<double4> cp(nc3 * ntags);
-  HIP_TRY(hipMemcpyAsync(nocup, s->cnt + R.cell_off[species], sizeof(int32_t) * nc3,
+  HIP_TRY(hipMemcpyAsync(occ_int.data(), s->cnt + R.cell_off[species], sizeof(int32_t) * nc3,
                          hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipMemcpyAsync(rs.data(), s->rs + R.rs_off[species], sizeof(int32_t) * rs.size(),
                          hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipMemcpyAsync(cp.data(), s->cpos + R.tag_off[species], sizeof(double4) * cp.size(),
                          hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
-  // packed rows back to the reference's ntags slots per cell
+  // device cells are z-fastest with packed rows; the reference's are x-fastest
+  // with ntags slots per cell
   for (int z = 0; z < n; z++)
     for (int y = 0; y < n; y++)
       for (int x = 0; x < n; x++) {
         const size_t c = (size_t)x + (size_t)n * (y + (size_t)n * z);
-        const size_t ri = ((size_t)x + (size_t)n * y) * (n + 1) + z;
+        const size_t row = (size_t)x + (size_t)n * y;
+        nocup[c] = occ_int[row * n + z];
+        const size_t ri = row * (n + 1) + z;
         const int s0 = rs[ri], occ = rs[ri + 1] - s0;
         for (int t = 0; t < ntags; t++)
           cell_vec[c * ntags + t] = t < occ ? (int32_t)cp[(size_t)s0 + t].w - sp_off : -1;
