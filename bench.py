@@ -34,10 +34,10 @@ B_BIN = 36                 # algorithmic bytes per particle, binning pass
 B_UPDATE = 120             # algorithmic bytes per particle, update pass (ring excluded)
 B_STEP = 228               # algorithmic bytes per particle-step (bin 36 + forces 72 + update 120)
 FLOP_PAIR = 22             # FP64 operations per ordered pair in the g(r) scan
-# HBM bytes per force launch (row walk, deferral depth 16) from the rocprofv3 FETCH_SIZE /
+# HBM bytes per force launch (slab walk, deferral depth 16) from the rocprofv3 FETCH_SIZE /
 # WRITE_SIZE runs in profiles/k_forces_traffic_pmc.txt, scaled by a copy kernel of known size
-TRAFFIC_FORCES_PLAIN = 6.46e6
-TRAFFIC_FORCES_BATCH = 1.491e8
+TRAFFIC_FORCES_PLAIN = 6.35e6
+TRAFFIC_FORCES_BATCH = 1.490e8
 
 
 def log(msg):

@@ -509,3 +509,26 @@ def test_persistent_steps_bit_identical(config1_positions):
     with pytest.raises(BDError):
         e.set_persistent(-1)
     e.close()
+
+
+def test_split_order_statistics_within_two_sigma(config1_positions):
+    """Past the first steps the split order (not bit-identical) must agree with
+    the reference order statistically: MSD and F_s(k=7) of a 16-replica
+    ensemble after 1500 steps within 2 sigma of the replica spread."""
+    side, x = config1_positions
+    params = [make_params(1000, 1000, side=side, T0=1.0, Tf=1.0, seed=700 + r)
+              for r in range(16)]
+    stats = {}
+    for order in (abi.BD_ORDER_REFERENCE, abi.BD_ORDER_SPLIT4):
+        e = Engine(params)
+        e.set_force_order(order)
+        for r in range(16):
+            e.set_positions(x, r)
+        e.mark_origin()
+        e.step(1500)
+        ob = [e.observables(7.0, r) for r in range(16)]
+        stats[order] = np.array([[o["msd_big"], o["msd_sml"], o["fself_big"]] for o in ob])
+        e.close()
+    a, b = stats[abi.BD_ORDER_REFERENCE], stats[abi.BD_ORDER_SPLIT4]
+    err = np.sqrt(a.var(0, ddof=1) / 16 + b.var(0, ddof=1) / 16)
+    assert np.all(np.abs(a.mean(0) - b.mean(0)) <= 2.0 * err + 1e-12), (a.mean(0), b.mean(0))
