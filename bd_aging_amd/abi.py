@@ -25,6 +25,8 @@ BD_ORDER_SPLIT4 = 4
 BD_ORDER_SPLIT8 = 8
 BD_WALK_ROWS = 0
 BD_WALK_SLABS = 1
+BD_WALK_CHUNK2 = 2
+BD_WALK_CHUNK4 = 3
 BD_TEMP_LINEAR = 0
 BD_TEMP_SINE = 1
 
@@ -98,6 +100,8 @@ SIGNATURES = {
     "bd_get_graph_steps": (C.c_int, [_P, _I]),
     "bd_set_walk": (C.c_int, [_P, C.c_int32]),
     "bd_get_walk": (C.c_int, [_P, _I]),
+    "bd_set_pass_split": (C.c_int, [_P, C.c_int32]),
+    "bd_get_pass_split": (C.c_int, [_P, _I]),
     "bd_set_defer_depth": (C.c_int, [_P, C.c_int32]),
     "bd_get_defer_depth": (C.c_int, [_P, _I]),
     "bd_set_brick": (C.c_int, [_P, C.c_int32]),

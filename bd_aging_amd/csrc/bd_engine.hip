@@ -826,7 +826,7 @@ __device__ inline double image_of(unsigned int code, double L) {
   return code == 0u ? 0.0 : (code == 1u ? -L : L);
 }
 
-template <int POT, int NR, int DEPTH>
+template <int POT, int NR, int DEPTH, bool REC = false>
 __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigma, double rc2,
                                   int self, double x, double y, double z, const int hw[3],
                                   const int* __restrict__ rs, const double4* __restrict__ cp,
@@ -938,10 +938,164 @@ __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigm
         if (DEPTH == 0) {
           pair_add<POT>(R, sigma, dist2, dx, dy, dz, fx, fy, fz, vir, pot);
         } else {
-          df.buf[df.nd * kBlk] = (unsigned int)sl | xbits |
-                                 (((mq >> 16) & 3u) << (kSlotBits + 2)) |
-                                 (((mq >> 18) & 3u) << (kSlotBits + 4)) | passbit;
-          if (++df.nd == DEPTH) drain<POT>(R, x, y, z, df, fx, fy, fz, vir, pot);
+          if (!REC || df.nd < DEPTH)
+            df.buf[df.nd * kBlk] = (unsigned int)sl | xbits |
+                                   (((mq >> 16) & 3u) << (kSlotBits + 2)) |
+                                   (((mq >> 18) & 3u) << (kSlotBits + 4)) | passbit;
+          if (++df.nd == DEPTH && !REC) drain<POT>(R, x, y, z, df, fx, fy, fz, vir, pot);
+        }
+      }
+    }
+  }
+}
+
+// Chunked slab walk: the slab walk's rows, ranges, order and arithmetic with
+// the memory latency batched.  The range bounds of all rows of a slab are
+// loaded before any is used (one latency per slab instead of one per row),
+// and the slab's candidate stream is loaded CH slots at a time before the
+// chunk is evaluated in stream order (one latency per CH candidates instead
+// of one per candidate).  A candidate equal to the particle itself can only
+// occur in its own cell, so the self test needs no centre-row flag.  The
+// deferral buffer is drained before a chunk that might overfill it, which
+// moves drains but not the order of the pair terms.  REC (both slab walks):
+// record the partners only, never drain; df.nd then counts every partner and
+// exceeds DEPTH when the column overflowed.
+template <int POT, int NR, int DEPTH, int CH, bool REC = false>
+__device__ inline void scan_chunks(const RepDev& R, int n, double cs, double sigma, double rc2,
+                                   int self, double x, double y, double z, const int hw[3],
+                                   const int* __restrict__ rs, const double4* __restrict__ cp,
+                                   Defer& df, unsigned int passbit, unsigned int* seg, double& fx,
+                                   double& fy, double& fz, double& vir, double& pot) {
+  static_assert(DEPTH == 0 || DEPTH >= CH, "a chunk must fit the deferral buffer");
+  constexpr int W = 2 * NR + 1;
+  const double L = R.side;
+  const double lim = rc2 + 1e-9;
+  const double ux = x - (double)hw[0] * cs, uy = y - (double)hw[1] * cs,
+               uz = z - (double)hw[2] * cs;
+  double g2lo[NR], g2hi[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    const double lo = slab_gap(-(k + 1), cs, uz);
+    const double hi = slab_gap(k + 1, cs, uz);
+    g2lo[k] = lo * lo;
+    g2hi[k] = hi * hi;
+  }
+  const double selfw = self >= 0 ? (double)self : -1.0;
+  for (int di = -NR; di <= NR; di++) {
+    const double gx = slab_gap(di, cs, ux);
+    const double gx2 = gx * gx;
+    if (gx2 > lim) continue;
+    const double shx = image_shift(hw[0] + di, n, L);
+    int a = hw[0] + di;
+    a = a < 0 ? a + n : (a >= n ? a - n : a);
+    // 1. every row's range bounds, loads issued back to back
+    int lo[W], hi[W], lo2[W], hi2[W];
+    unsigned int code[W];  // y image << 4 | z image of range b << 2 | z image of range a
+#pragma unroll
+    for (int r = 0; r < W; r++) {
+      const int dj = r - NR;
+      const double gy = slab_gap(dj, cs, uy);
+      const double gxy2 = gx2 + gy * gy;
+      int b = hw[1] + dj;
+      b = b < 0 ? b + n : (b >= n ? b - n : b);
+      int klo = 0, khi = 0;
+#pragma unroll
+      for (int k = 0; k < NR; k++) {
+        if (gxy2 + g2lo[k] <= lim) klo = -(k + 1);
+        if (gxy2 + g2hi[k] <= lim) khi = k + 1;
+      }
+      const int* rrow = rs + (a + n * b) * (n + 1);
+      const int z0 = hw[2] + klo, z1 = hw[2] + khi;
+      const bool wlo = z0 < 0, whi = z1 >= n;
+      lo[r] = hi[r] = lo2[r] = hi2[r] = 0;
+      if (gxy2 <= lim) {
+        lo[r] = rrow[wlo ? z0 + n : z0];
+        hi[r] = rrow[(wlo || whi) ? n : z1 + 1];
+        if (wlo || whi) {
+          lo2[r] = rrow[0];
+          hi2[r] = rrow[wlo ? z1 + 1 : z1 - n + 1];
+        }
+      }
+      code[r] = (image_code(image_shift(hw[1] + dj, n, L)) << 4) | (whi ? 8u : 0u) |
+                (wlo ? 1u : 0u);
+    }
+    // 2. the slab's non-empty ranges in visiting order
+    int nseg = 0;
+#pragma unroll
+    for (int r = 0; r < W; r++) {
+      const unsigned int ycode = code[r] >> 4;
+      if (hi[r] > lo[r]) {
+        seg[(2 * nseg) * kBlk] = (unsigned int)lo[r];
+        seg[(2 * nseg + 1) * kBlk] =
+            (unsigned int)(hi[r] - lo[r]) | (ycode << 16) | ((code[r] & 3u) << 18);
+        nseg++;
+      }
+      if (hi2[r] > lo2[r]) {
+        seg[(2 * nseg) * kBlk] = (unsigned int)lo2[r];
+        seg[(2 * nseg + 1) * kBlk] =
+            (unsigned int)(hi2[r] - lo2[r]) | (ycode << 16) | (((code[r] >> 2) & 3u) << 18);
+        nseg++;
+      }
+    }
+    // 3. the stream, CH candidates per round; the next range's descriptor is
+    //    read one range ahead
+    const unsigned int xbits = image_code(shx) << kSlotBits;
+    int k = 0, st = 0, rem = 0;
+    unsigned int mc = 0u, nst = 0u, nmc = 0u;
+    double sy = 0.0, sz = 0.0;
+    if (nseg > 0) {
+      nst = seg[0];
+      nmc = seg[kBlk];
+    }
+    while (true) {
+      int sl[CH];
+      unsigned int mu[CH];
+      double syu[CH], szu[CH];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        if (rem == 0 && k < nseg) {
+          st = (int)nst;
+          mc = nmc;
+          rem = (int)(mc & 0xffffu);
+          sy = image_of((mc >> 16) & 3u, L);
+          sz = image_of((mc >> 18) & 3u, L);
+          k++;
+          if (k < nseg) {
+            nst = seg[(2 * k) * kBlk];
+            nmc = seg[(2 * k + 1) * kBlk];
+          }
+        }
+        sl[u] = rem > 0 ? st : -1;
+        mu[u] = mc;
+        syu[u] = sy;
+        szu[u] = sz;
+        if (rem > 0) {
+          st++;
+          rem--;
+        }
+      }
+      if (sl[0] < 0) break;
+      double4 q[CH];
+#pragma unroll
+      for (int u = 0; u < CH; u++) q[u] = cp[sl[u] >= 0 ? sl[u] : sl[0]];
+      if (DEPTH > 0 && !REC && df.nd > DEPTH - CH)
+        drain<POT>(R, x, y, z, df, fx, fy, fz, vir, pot);
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const double dx = (q[u].x - x) + shx;
+        const double dy = (q[u].y - y) + syu[u];
+        const double dz = (q[u].z - z) + szu[u];
+        const double dist2 = dx * dx + dy * dy + dz * dz;
+        if (sl[u] >= 0 && dist2 < rc2 && q[u].w != selfw) {
+          if (DEPTH == 0) {
+            pair_add<POT>(R, sigma, dist2, dx, dy, dz, fx, fy, fz, vir, pot);
+          } else {
+            if (!REC || df.nd < DEPTH)
+              df.buf[df.nd * kBlk] = (unsigned int)sl[u] | xbits |
+                                     (((mu[u] >> 16) & 3u) << (kSlotBits + 2)) |
+                                     (((mu[u] >> 18) & 3u) << (kSlotBits + 4)) | passbit;
+            df.nd++;
+          }
         }
       }
     }
@@ -991,6 +1145,14 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
         else
           scan_slabs<POT, 2, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df,
                                     pass == 0 ? 0u : kPassBit, seg, fx, fy, fz, v, u);
+      } else if (WALK >= 2) {
+        constexpr int CH = WALK == 2 ? 2 : 4;
+        if (nr == 1)
+          scan_chunks<POT, 1, DEPTH, CH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df,
+                                         pass == 0 ? 0u : kPassBit, seg, fx, fy, fz, v, u);
+        else
+          scan_chunks<POT, 2, DEPTH, CH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df,
+                                         pass == 0 ? 0u : kPassBit, seg, fx, fy, fz, v, u);
       } else if (nr == 1) {
         scan_rows<POT, 1, DEPTH>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df,
                                  pass == 0 ? 0u : kPassBit, fx, fy, fz, v, u);
@@ -1011,7 +1173,7 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
 // then cross-species stencil (main.cu:576-585), accumulators start at zero.
 // NEIGH records the interacting partners in traversal order (test hook).
 // DEPTH: per-thread LDS entries for deferred pair terms (0 = direct);
-// WALK: 0 row walk, 1 slab walk.
+// WALK: 0 row walk, 1 slab walk, 2 / 3 chunked slab walk (2 / 4 candidates per load round).
 template <int POT, bool NEIGH, int DEPTH, int WALK>
 __device__ __forceinline__ void k_forces_block(
     int vb, const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
@@ -1022,7 +1184,7 @@ __device__ __forceinline__ void k_forces_block(
              double* __restrict__ pot, int* __restrict__ ncounts, int* __restrict__ nlist,
              int maxn) {
   __shared__ unsigned int dbuf[DEPTH > 0 ? DEPTH * kBlk : 1];
-  __shared__ unsigned int sbuf[WALK == 1 ? 2 * kSlabSegs * kBlk : 1];
+  __shared__ unsigned int sbuf[WALK >= 1 ? 2 * kSlabSegs * kBlk : 1];
   const int4 t = tbl[vb];
   if (t.x < 0) return;
   const int i = t.y + (int)threadIdx.x;
@@ -1062,6 +1224,129 @@ __global__ void __launch_bounds__(kBlk)
              int maxn) {
   k_forces_block<POT, NEIGH, DEPTH, WALK>((int)blockIdx.x, reps, tbl, px, py, pz, rs, cpos, ffx,
                                           ffy, ffz, vir, pot, ncounts, nlist, maxn);
+}
+
+// One stencil pass (0: same species, 1: the other species, main.cu:576-585)
+// of replica-local particle i with a slab walk (WALK 1..3), on grids of at
+// least 2 nrange + 3 cells.  Returns false when the particle has no cell in
+// that grid.  REC: record the partners in df only (see scan_chunks).
+template <int POT, int DEPTH, int WALK, bool REC>
+__device__ inline bool walk_pass(const RepDev& R, int i, int pass, double x, double y, double z,
+                                 const int* __restrict__ rs, const double4* __restrict__ cpos,
+                                 Defer& df, unsigned int* seg, double& fx, double& fy,
+                                 double& fz, double& v, double& u) {
+  const int g = i < R.nb ? 0 : 1;
+  const int gg = pass == 0 ? g : 1 - g;
+  const int nr = pass == 0 ? R.nrange_same[g] : R.nrange_cross[g];
+  const double sigma = pass == 0 ? R.sigma[g] : R.sigma_bs;
+  const double rc2 = pass == 0 ? R.rc2_same[g] : R.rc2_bs;
+  const int self = pass == 0 ? i : -1;
+  int hg[3];
+  if (!cell_coords(R, gg, x, y, z, hg[0], hg[1], hg[2])) return false;
+  const int n = R.ncell[gg];
+  const double cs = R.cell_side[gg];
+  const int* rs_g = rs + R.rs_off[gg];
+  const double4* cp_g = cpos + R.tag_off[gg];
+  const unsigned int pb = pass == 0 ? 0u : kPassBit;
+  if (WALK == 1) {
+    if (nr == 1)
+      scan_slabs<POT, 1, DEPTH, REC>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df, pb,
+                                     seg, fx, fy, fz, v, u);
+    else
+      scan_slabs<POT, 2, DEPTH, REC>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df, pb,
+                                     seg, fx, fy, fz, v, u);
+  } else {
+    constexpr int CH = WALK == 2 ? 2 : 4;
+    if (nr == 1)
+      scan_chunks<POT, 1, DEPTH, CH, REC>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g,
+                                          df, pb, seg, fx, fy, fz, v, u);
+    else
+      scan_chunks<POT, 2, DEPTH, CH, REC>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g,
+                                          df, pb, seg, fx, fy, fz, v, u);
+  }
+  return true;
+}
+
+// Stencil passes split over two wavefronts (no reference counterpart): for
+// the same 64 particles, wavefront 0 walks the same-species stencil and
+// evaluates its pair terms, while wavefront 1 walks the cross-species stencil
+// and only records its partners in LDS; after the barrier wavefront 0
+// evaluates those records after its own, so every pair term is added in the
+// single-wavefront kernel's order (bit-identical).  A particle whose
+// cross-species partners overflow the DEPTH-entry column has that pass walked
+// again by wavefront 0.  Twice the wavefronts of k_forces for the same work,
+// for systems too small to fill the GPU with one lane per particle.
+template <int POT, int DEPTH, int WALK>
+__global__ void __launch_bounds__(2 * kBlk)
+    k_forces_pair(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+                  const double* __restrict__ px, const double* __restrict__ py,
+                  const double* __restrict__ pz, const int* __restrict__ rs,
+                  const double4* __restrict__ cpos, double* __restrict__ ffx,
+                  double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
+                  double* __restrict__ pot) {
+  static_assert(DEPTH > 0 && WALK >= 1, "the split needs a slab walk with deferral");
+  __shared__ unsigned int dbuf[2 * DEPTH * kBlk];
+  __shared__ unsigned int sbuf[2 * 2 * kSlabSegs * kBlk];
+  __shared__ int n1s[kBlk];
+  const int4 t = tbl[blockIdx.x];
+  if (t.x < 0) return;
+  const int lane = (int)threadIdx.x % kBlk;
+  const int w = (int)threadIdx.x / kBlk;
+  const int i = t.y + lane;
+  const bool live = i < t.z;
+  const RepDev& R = reps[t.x];
+  const int p = R.p_off + i;
+  const int g = i < R.nb ? 0 : 1;
+  double x = 0.0, y = 0.0, z = 0.0;
+  if (live) {
+    x = px[p];
+    y = py[p];
+    z = pz[p];
+  }
+  double fx = 0.0, fy = 0.0, fz = 0.0, v = 0.0, u = 0.0;
+  Defer df{cpos + R.tag_off[g], cpos + R.tag_off[1 - g], R.sigma[g], R.sigma_bs,
+           dbuf + w * DEPTH * kBlk + lane, 0};
+  unsigned int* seg = sbuf + w * 2 * kSlabSegs * kBlk + lane;
+  bool ok = true;
+  if (w == 1) {
+    int n1 = -1;  // -1: wavefront 0 walks the pass itself; -2: no cell in that grid
+    if (live && R.fast) {
+      if (walk_pass<POT, DEPTH, WALK, true>(R, i, 1, x, y, z, rs, cpos, df, seg, fx, fy, fz, v,
+                                            u))
+        n1 = df.nd <= DEPTH ? df.nd : -1;
+      else
+        n1 = -2;
+    }
+    n1s[lane] = n1;
+  } else if (live) {
+    if (R.fast)
+      ok = walk_pass<POT, DEPTH, WALK, false>(R, i, 0, x, y, z, rs, cpos, df, seg, fx, fy, fz, v,
+                                              u);
+    else
+      forces_global<POT, false>(R, i, x, y, z, rs, cpos, fx, fy, fz, v, u);
+  }
+  __syncthreads();
+  if (w != 0 || !live) return;
+  if (R.fast && ok) {
+    // as the single-wavefront walk: a particle without a cell in the second
+    // grid ends its walk there, before the final drain
+    const int n1 = n1s[lane];
+    if (n1 != -2) {
+      drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
+      if (n1 >= 0) {
+        Defer d1{df.cp0, df.cp1, df.sigma0, df.sigma1, dbuf + DEPTH * kBlk + lane, n1};
+        drain<POT>(R, x, y, z, d1, fx, fy, fz, v, u);
+      } else if (walk_pass<POT, DEPTH, WALK, false>(R, i, 1, x, y, z, rs, cpos, df, seg, fx, fy,
+                                                    fz, v, u)) {
+        drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
+      }
+    }
+  }
+  ffx[p] = fx;
+  ffy[p] = fy;
+  ffz[p] = fz;
+  vir[p] = v;
+  pot[p] = u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1654,8 +1939,11 @@ struct bd_sys {
   int order = 1;
   // deferred pair terms of the walk: 0 (direct), 16 or 32 LDS entries
   int defer = 16;
-  // walk of the default force kernel: 0 rows, 1 slabs
+  // walk of the default force kernel: 0 rows, 1 slabs, 2 / 3 chunked slabs
   int walk = 1;
+  // stencil passes of the thread-per-particle kernel: 0 one wavefront, 1 the
+  // two passes on two wavefronts (slab walks with deferral only)
+  int pass_split = 0;
   bool defer_ok = true;  // every grid's slot index fits the 26-bit record
   std::vector<int4> tbl_split_h;
   int4* tbl_split = nullptr;
@@ -1718,12 +2006,54 @@ void launch_forces_w(bd_sys* s, const int4* tbl, int nblk) {
     launch_forces_k<false, 32, WALK>(s, nullptr, nullptr, 0, tbl, nblk);
 }
 
+template <int DEPTH, int WALK>
+void launch_pair_k(bd_sys* s, const int4* tbl, int nblk) {
+  switch (s->pot_kind) {
+    case BD_POT_WCA:
+      k_forces_pair<BD_POT_WCA, DEPTH, WALK><<<nblk, 2 * kBlk, 0, s->stream>>>(
+          s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
+          s->pot);
+      break;
+    case BD_POT_WCA_MODIFIED:
+      k_forces_pair<BD_POT_WCA_MODIFIED, DEPTH, WALK><<<nblk, 2 * kBlk, 0, s->stream>>>(
+          s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
+          s->pot);
+      break;
+    default:
+      k_forces_pair<BD_POT_HARMONIC, DEPTH, WALK><<<nblk, 2 * kBlk, 0, s->stream>>>(
+          s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
+          s->pot);
+      break;
+  }
+}
+
+template <int WALK>
+void launch_pair_w(bd_sys* s, const int4* tbl, int nblk) {
+  if (s->defer == 16)
+    launch_pair_k<16, WALK>(s, tbl, nblk);
+  else
+    launch_pair_k<32, WALK>(s, tbl, nblk);
+}
+
 template <bool NEIGH>
 void launch_forces_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl, int nblk) {
+  if (!NEIGH && s->pass_split && s->defer > 0 && s->walk >= 1) {
+    if (s->walk == 1)
+      launch_pair_w<1>(s, tbl, nblk);
+    else if (s->walk == 2)
+      launch_pair_w<2>(s, tbl, nblk);
+    else
+      launch_pair_w<3>(s, tbl, nblk);
+    return;
+  }
   if (NEIGH)
     launch_forces_k<true, 0, 0>(s, ncounts, nlist, maxn, tbl, nblk);
   else if (s->walk == 1)
     launch_forces_w<1>(s, tbl, nblk);
+  else if (s->walk == 2)
+    launch_forces_w<2>(s, tbl, nblk);
+  else if (s->walk == 3)
+    launch_forces_w<3>(s, tbl, nblk);
   else
     launch_forces_w<0>(s, tbl, nblk);
 }
@@ -2634,16 +2964,22 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
   const std::string key = tune_key(s);
   if (cache_path) {
     if (FILE* f = std::fopen(cache_path, "r")) {
-      char k[200];
-      int b, d;
-      int found_b = -2, found_d = 0;
-      while (std::fscanf(f, "%199s %d %d", k, &b, &d) == 3)
-        if (key == k) found_b = b, found_d = d;
+      // lines: key brick depth walk split (a line without walk: brick -1 =
+      // slab walk; without split: one wavefront)
+      char line[320], k[200];
+      int found_b = -2, found_d = 0, found_w = 0, found_p = 0;
+      while (std::fgets(line, sizeof line, f)) {
+        int b, d, w, sp = 0;
+        const int nf = std::sscanf(line, "%199s %d %d %d %d", k, &b, &d, &w, &sp);
+        if (nf < 3 || key != k) continue;
+        if (nf == 3) w = b < 0 ? BD_WALK_SLABS : BD_WALK_ROWS;
+        found_b = b < 0 ? 0 : b, found_d = d, found_w = w, found_p = nf >= 5 ? sp : 0;
+      }
       std::fclose(f);
-      if (found_b >= -1 && bd_set_defer_depth(s, found_d) == BD_OK &&
-          bd_set_walk(s, found_b < 0 ? BD_WALK_SLABS : BD_WALK_ROWS) == BD_OK &&
-          bd_set_brick(s, found_b < 0 ? 0 : found_b) == BD_OK) {
-        if (chosen) *chosen = found_b < 0 ? 0 : found_b;
+      if (found_b >= 0 && bd_set_defer_depth(s, found_d) == BD_OK &&
+          bd_set_walk(s, found_w) == BD_OK && bd_set_pass_split(s, found_p) == BD_OK &&
+          bd_set_brick(s, found_b) == BD_OK) {
+        if (chosen) *chosen = found_b;
         return BD_OK;
       }
     }
@@ -2661,25 +2997,35 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
     s->graph = nullptr;
     s->graph_steps = 0;
   }
-  // kernel shapes: the row walk at three deferral depths, the LDS bricks
+  // kernel shapes: the row, slab and chunked slab walks at three deferral
+  // depths, the LDS bricks
   struct Shape {
-    int brick, defer;  // brick -1: the slab walk (brick 0 with walk 1)
+    int brick, defer, walk, split;
   };
-  const Shape candidates[] = {{0, 0},  {0, 16},  {0, 32},  {-1, 0}, {-1, 16},
-                              {-1, 32}, {1, 0},   {2, 0},   {3, 0},   {4, 0}};
-  Shape best{0, 0};
+  const Shape candidates[] = {
+      {0, 0, BD_WALK_ROWS, 0},    {0, 16, BD_WALK_ROWS, 0},   {0, 32, BD_WALK_ROWS, 0},
+      {0, 0, BD_WALK_SLABS, 0},   {0, 16, BD_WALK_SLABS, 0},  {0, 32, BD_WALK_SLABS, 0},
+      {0, 0, BD_WALK_CHUNK2, 0},  {0, 16, BD_WALK_CHUNK2, 0}, {0, 32, BD_WALK_CHUNK2, 0},
+      {0, 0, BD_WALK_CHUNK4, 0},  {0, 16, BD_WALK_CHUNK4, 0}, {0, 32, BD_WALK_CHUNK4, 0},
+      {0, 16, BD_WALK_SLABS, 1},  {0, 32, BD_WALK_SLABS, 1},  {0, 16, BD_WALK_CHUNK2, 1},
+      {0, 32, BD_WALK_CHUNK2, 1}, {0, 16, BD_WALK_CHUNK4, 1}, {0, 32, BD_WALK_CHUNK4, 1},
+      {1, 0, BD_WALK_ROWS, 0},    {2, 0, BD_WALK_ROWS, 0},    {3, 0, BD_WALK_ROWS, 0},
+      {4, 0, BD_WALK_ROWS, 0}};
+  Shape best{0, 0, BD_WALK_ROWS, 0};
   double best_ms = 1e300;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  const Shape keep{s->walk == 1 ? -1 : s->brick, s->defer};
+  const Shape keep{s->brick, s->defer, s->walk, s->pass_split};
   for (const Shape& c : candidates) {
-    if (s->order != BD_ORDER_REFERENCE && (c.brick != keep.brick || c.defer != keep.defer))
+    if (s->order != BD_ORDER_REFERENCE && (c.brick != keep.brick || c.defer != keep.defer ||
+                                           c.walk != keep.walk || c.split != keep.split))
       continue;
     if (c.defer != 0 && !s->defer_ok) continue;
-    if (setup_bricks(s, c.brick < 0 ? 0 : c.brick) != BD_OK) continue;
-    s->walk = c.brick < 0 ? 1 : 0;
+    if (setup_bricks(s, c.brick) != BD_OK) continue;
+    s->walk = c.walk;
     s->defer = c.defer;
+    s->pass_split = c.split;
     launch_forces(s);  // warm-up
     HIP_TRY(hipEventRecord(e0, s->stream));
     for (int k = 0; k < trial_launches; k++) launch_forces(s);
@@ -2696,27 +3042,31 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
   (void)hipEventDestroy(e1);
   g_err.clear();
   s->defer = best.defer;
-  s->walk = best.brick < 0 ? 1 : 0;
-  rc = bd_set_brick(s, best.brick < 0 ? 0 : best.brick);
+  s->walk = best.walk;
+  s->pass_split = best.split;
+  rc = bd_set_brick(s, best.brick);
   if (rc != BD_OK) {
     s->defer = keep.defer;
-    s->walk = keep.brick < 0 ? 1 : 0;
-    bd_set_brick(s, keep.brick < 0 ? 0 : keep.brick);
+    s->walk = keep.walk;
+    s->pass_split = keep.split;
+    bd_set_brick(s, keep.brick);
     return rc;
   }
   if (cache_path)
     if (FILE* f = std::fopen(cache_path, "a")) {
-      std::fprintf(f, "%s %d %d\n", key.c_str(), best.brick, best.defer);
+      std::fprintf(f, "%s %d %d %d %d\n", key.c_str(), best.brick, best.defer, best.walk,
+                   best.split);
       std::fclose(f);
     }
-  if (chosen) *chosen = best.brick < 0 ? 0 : best.brick;
+  if (chosen) *chosen = best.brick;
   return BD_OK;
 }
 
 int bd_set_walk(bd_sys* s, int32_t walk) {
   if (!s) return fail(BD_E_ARG, "null handle");
-  if (walk != BD_WALK_ROWS && walk != BD_WALK_SLABS)
-    return fail(BD_E_ARG, "walk must be BD_WALK_ROWS or BD_WALK_SLABS");
+  if (walk != BD_WALK_ROWS && walk != BD_WALK_SLABS && walk != BD_WALK_CHUNK2 &&
+      walk != BD_WALK_CHUNK4)
+    return fail(BD_E_ARG, "walk must be BD_WALK_ROWS, _SLABS, _CHUNK2 or _CHUNK4");
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (s->graph) {
     (void)hipGraphExecDestroy(s->graph);
@@ -2730,6 +3080,25 @@ int bd_set_walk(bd_sys* s, int32_t walk) {
 int bd_get_walk(const bd_sys* s, int32_t* walk) {
   if (!s || !walk) return fail(BD_E_ARG, "null argument");
   *walk = s->walk;
+  return BD_OK;
+}
+
+int bd_set_pass_split(bd_sys* s, int32_t split) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (split != 0 && split != 1) return fail(BD_E_ARG, "pass split must be 0 or 1");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->graph) {
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_steps = 0;
+  }
+  s->pass_split = split;
+  return BD_OK;
+}
+
+int bd_get_pass_split(const bd_sys* s, int32_t* split) {
+  if (!s || !split) return fail(BD_E_ARG, "null argument");
+  *split = s->pass_split;
   return BD_OK;
 }
 

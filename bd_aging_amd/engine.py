@@ -207,12 +207,22 @@ class Engine:
         return v.value
 
     def set_walk(self, walk):
-        """Force walk: abi.BD_WALK_ROWS (0) or abi.BD_WALK_SLABS (1)."""
+        """Force walk: abi.BD_WALK_ROWS (0), _SLABS (1), _CHUNK2 (2) or _CHUNK4 (3)."""
         check(self.lib.bd_set_walk(self.h, int(walk)))
 
     def walk(self):
         v = C.c_int32()
         check(self.lib.bd_get_walk(self.h, C.byref(v)))
+        return v.value
+
+    def set_pass_split(self, split):
+        """1: the cross-species stencil pass on a second wavefront (slab walks
+        with deferral); 0: one wavefront per particle."""
+        check(self.lib.bd_set_pass_split(self.h, int(split)))
+
+    def pass_split(self):
+        v = C.c_int32()
+        check(self.lib.bd_get_pass_split(self.h, C.byref(v)))
         return v.value
 
     def set_defer_depth(self, depth):

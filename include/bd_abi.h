@@ -171,18 +171,32 @@ int bd_get_defer_depth(const bd_sys* s, int32_t* depth);
  * BD_WALK_ROWS loops over each stencil row's slot range; BD_WALK_SLABS
  * (default) lists the ranges of all rows with one di in LDS and loops over
  * them as one stream (fewer idle lanes when range lengths differ between
- * particles).  Same pairs, order and arithmetic: bit-identical. */
-enum { BD_WALK_ROWS = 0, BD_WALK_SLABS = 1 };
+ * particles); BD_WALK_CHUNK2 / _CHUNK4 walk the same stream with the range
+ * bounds of a slab loaded together and the candidates loaded 2 / 4 at a
+ * time (fewer exposed memory latencies).  Same pairs, order and arithmetic:
+ * bit-identical. */
+enum { BD_WALK_ROWS = 0, BD_WALK_SLABS = 1, BD_WALK_CHUNK2 = 2, BD_WALK_CHUNK4 = 3 };
 int bd_set_walk(bd_sys* s, int32_t walk);
 int bd_get_walk(const bd_sys* s, int32_t* walk);
 
+/* Stencil passes over wavefronts (no reference counterpart): 0 (default)
+ * walks both passes of a particle (main.cu:576-585) in one lane; 1 walks the
+ * cross-species pass on a second wavefront at the same time, which only
+ * records its partners for the first wavefront to evaluate after its own
+ * (twice the wavefronts, for systems too small to fill the GPU).  Takes
+ * effect with a slab walk (BD_WALK_SLABS, _CHUNK2, _CHUNK4) at a nonzero
+ * deferral depth.  Bit-identical.  BD_E_ARG for values other than 0, 1. */
+int bd_set_pass_split(bd_sys* s, int32_t split);
+int bd_get_pass_split(const bd_sys* s, int32_t* split);
+
 /* Autotuner (no reference counterpart): measures the force kernel for every
- * shape (row and slab walks at deferral depths 0, 16 and 32; bricks 1..4;
- * all bit-identical by construction) with forces-only launches on the
- * current state, keeps the fastest and returns its brick in *chosen (the
- * walk and depth are read with bd_get_walk / bd_get_defer_depth).  With
+ * shape (row, slab and chunked slab walks at deferral depths 0, 16 and 32,
+ * the slab walks also with the pass split; bricks 1..4; all bit-identical by
+ * construction) with forces-only launches on the current state, keeps the
+ * fastest and returns its brick in *chosen (the walk, depth and split are
+ * read with bd_get_walk / bd_get_defer_depth / bd_get_pass_split).  With
  * cache_path != NULL the choice is looked up in / appended to that text
- * file as "key brick depth" (brick -1 = slab walk), keyed by (replicas, N,
+ * file as "key brick depth walk split", keyed by (replicas, N,
  * phi, potential, order). */
 int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32_t* chosen);
 

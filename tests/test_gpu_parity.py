@@ -136,25 +136,68 @@ def test_every_brick_shape_bit_identical(config1_positions, pot):
             continue
         tried.append(b)
         assert e.brick() == b
-        for walk in ((abi.BD_WALK_ROWS, abi.BD_WALK_SLABS) if b == 0 else (abi.BD_WALK_ROWS,)):
+        walks = (abi.BD_WALK_ROWS, abi.BD_WALK_SLABS, abi.BD_WALK_CHUNK2, abi.BD_WALK_CHUNK4)
+        for walk in (walks if b == 0 else (abi.BD_WALK_ROWS,)):
             e.set_walk(walk)
             assert e.walk() == walk
             for depth in ((0, 16, 32) if b == 0 else (16,)):
                 e.set_defer_depth(depth)
                 assert e.defer_depth() == depth
-                e.compute_forces()
-                fe, ve, ue = e.forces()
-                assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and \
-                    np.array_equal(ue, uo), (b, walk, depth)
+                for split in ((0, 1) if b == 0 and walk != abi.BD_WALK_ROWS else (0,)):
+                    e.set_pass_split(split)
+                    assert e.pass_split() == split
+                    e.compute_forces()
+                    fe, ve, ue = e.forces()
+                    assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and \
+                        np.array_equal(ue, uo), (b, walk, depth, split)
+                e.set_pass_split(0)
         e.set_walk(abi.BD_WALK_ROWS)
     assert {0, 1, 2, 3} <= set(tried)
     for bad in (-1, 8, 64):
         with pytest.raises(BDError):
             e.set_defer_depth(bad)
-    for bad in (-1, 2):
+    for bad in (-1, 4):
         with pytest.raises(BDError):
             e.set_walk(bad)
+    for bad in (-1, 2):
+        with pytest.raises(BDError):
+            e.set_pass_split(bad)
     e.close()
+
+
+def test_pass_split_column_overflow(config1_positions):
+    """A big particle with more cross-species partners than the deferral
+    column holds (40 small particles packed around it): the split kernel
+    walks that pass again on the first wavefront, bit-identical."""
+    side, x = config1_positions
+    x = x.copy()
+    rng = np.random.default_rng(5)
+    k = 40
+    d = rng.normal(size=(k, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    x[1000:1000 + k] = (x[0] + d * rng.uniform(0.7, 1.3, k)[:, None]) % side
+    p = make_params(1000, 1000, side=side, seed=3, ntags_big=24, ntags_sml=24)
+    o = orc.System(p)
+    o.set_positions(x)
+    assert o.cell_locate() == 0
+    o.compute_forces()
+    fo, vo, uo = o.forces()
+    counts, _ = o.neighbours(128)
+    assert counts[0] > 32
+    e = Engine(p)
+    e.set_positions(x)
+    e.cell_locate()
+    for walk in (abi.BD_WALK_SLABS, abi.BD_WALK_CHUNK2, abi.BD_WALK_CHUNK4):
+        for depth in (16, 32):
+            e.set_walk(walk)
+            e.set_defer_depth(depth)
+            e.set_pass_split(1)
+            e.compute_forces()
+            fe, ve, ue = e.forces()
+            assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and \
+                np.array_equal(ue, uo), (walk, depth)
+    e.close()
+    o.close()
 
 
 @pytest.fixture(scope="module")
@@ -176,16 +219,20 @@ def test_forces_bit_exact_at_bench_size(plain_positions):
     e = Engine(p)
     e.set_positions(x)
     e.cell_locate()
-    shapes = [(0, 0, 0), (0, 16, 0), (0, 32, 0), (0, 0, 1), (0, 16, 1), (0, 32, 1), (1, 16, 0),
-              (2, 16, 0)]
-    for b, depth, walk in shapes:
+    shapes = [(0, 0, 0, 0), (0, 16, 0, 0), (0, 32, 0, 0), (0, 0, 1, 0), (0, 16, 1, 0),
+              (0, 32, 1, 0), (0, 0, 2, 0), (0, 16, 2, 0), (0, 32, 2, 0), (0, 0, 3, 0),
+              (0, 16, 3, 0), (0, 32, 3, 0), (0, 16, 1, 1), (0, 32, 1, 1), (0, 16, 2, 1),
+              (0, 32, 2, 1), (0, 16, 3, 1), (0, 32, 3, 1), (1, 16, 0, 0), (2, 16, 0, 0)]
+    for b, depth, walk, split in shapes:
         e.set_brick(b)
         e.set_defer_depth(depth)
         e.set_walk(walk)
+        e.set_pass_split(split)
         e.compute_forces()
         fe, ve, ue = e.forces()
         assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and np.array_equal(ue, uo), \
-            (b, depth, walk)
+            (b, depth, walk, split)
+    e.set_pass_split(0)
     for sp in (0, 1):
         ne, ce = e.cells(sp)
         no, co = o.cells(sp)
@@ -435,8 +482,7 @@ def test_autotune_keeps_results_and_caches(config1_positions, tmp_path):
     f1 = e.forces()
     assert all(np.array_equal(a, c) for a, c in zip(f0, f1))
     last = open(cache).read().split()
-    assert last[-2] == str(-1 if e.walk() == abi.BD_WALK_SLABS else b)
-    assert last[-1] == str(e.defer_depth())
+    assert last[-4:] == [str(b), str(e.defer_depth()), str(e.walk()), str(e.pass_split())]
     e2 = Engine(p)
     e2.set_positions(x)
     assert e2.autotune(5, cache) == b  # served from the cache
