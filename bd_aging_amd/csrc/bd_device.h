@@ -124,14 +124,15 @@ __device__ inline void gauss3(uint64_t d0, uint64_t d1, uint64_t d2, uint64_t d3
   z = r2 * c;
 }
 
-// Pair potentials (potentials.cu:54-76, FP64).  nf is dU/dr.
+// Pair potentials (potentials.cu:54-76, FP64): e is the pair's energy term,
+// added to the running potential as pot = pot + e; nf is dU/dr.
 template <int KIND>
-__device__ inline void potential(double kappa, double sigma, double dist_inv, double& pot,
-                                 double& nf) {
+__device__ inline void pair_energy(double kappa, double sigma, double dist_inv, double& e,
+                                   double& nf) {
   if (KIND == BD_POT_HARMONIC) {
     double r = 1.0 / dist_inv;
     double delta = sigma - r;
-    pot = pot + (0.5 * kappa) * delta * delta;
+    e = (0.5 * kappa) * delta * delta;
     nf = -kappa * delta;
   } else {
     double sor = dist_inv * sigma;
@@ -139,13 +140,21 @@ __device__ inline void potential(double kappa, double sigma, double dist_inv, do
     sor6 = sor6 * sor6;
     double sor12 = sor6 * sor6;
     if (KIND == BD_POT_WCA) {
-      pot = pot + (4.0 * (sor12 - sor6) + 1.0);
+      e = 4.0 * (sor12 - sor6) + 1.0;
       nf = -4.0 * (12.0 * sor12 - 6.0 * sor6) * dist_inv;
     } else {
-      pot = pot + ((sor12 - 2.0 * sor6) + 1.0);
+      e = (sor12 - 2.0 * sor6) + 1.0;
       nf = -12.0 * (sor12 - sor6) * dist_inv;
     }
   }
+}
+
+template <int KIND>
+__device__ inline void potential(double kappa, double sigma, double dist_inv, double& pot,
+                                 double& nf) {
+  double e;
+  pair_energy<KIND>(kappa, sigma, dist_inv, e, nf);
+  pot = pot + e;
 }
 
 }  // namespace bd

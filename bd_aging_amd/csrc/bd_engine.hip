@@ -152,8 +152,10 @@ __device__ inline void insert_cell(const RepDev& R, int rep, int i, double x, do
                                    double z, int* nocup, int* ins, int* flags) {
   const int g = i < R.nb ? 0 : 1;
   const int loc = g == 0 ? i : i - R.nb;
-  int ii, jj, kk;
-  if (!cell_coords(R, g, x, y, z, ii, jj, kk)) {
+  int ii, jj, kk, io, jo, ko;
+  // a position the force walk cannot place in both grids (non-finite, or far
+  // outside the box) is an error of the whole replica
+  if (!cell_coords(R, g, x, y, z, ii, jj, kk) || !cell_coords(R, 1 - g, x, y, z, io, jo, ko)) {
     atomicOr(&flags[rep], kFlagNonFinite);
     return;
   }
@@ -831,7 +833,8 @@ __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigm
                                   int self, double x, double y, double z, const int hw[3],
                                   const int* __restrict__ rs, const double4* __restrict__ cp,
                                   Defer& df, unsigned int passbit, unsigned int* seg, double& fx,
-                                  double& fy, double& fz, double& vir, double& pot) {
+                                  double& fy, double& fz, double& vir, double& pot,
+                                  int di0 = -NR, int di1 = NR) {
   const double L = R.side;
   const double lim = rc2 + 1e-9;
   const double ux = x - (double)hw[0] * cs, uy = y - (double)hw[1] * cs,
@@ -845,7 +848,7 @@ __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigm
     g2hi[k] = hi * hi;
   }
   const double selfw = self >= 0 ? (double)self : -1.0;
-  for (int di = -NR; di <= NR; di++) {
+  for (int di = di0; di <= di1; di++) {
     const double gx = slab_gap(di, cs, ux);
     const double gx2 = gx * gx;
     const double shx = image_shift(hw[0] + di, n, L);
@@ -965,7 +968,8 @@ __device__ inline void scan_chunks(const RepDev& R, int n, double cs, double sig
                                    int self, double x, double y, double z, const int hw[3],
                                    const int* __restrict__ rs, const double4* __restrict__ cp,
                                    Defer& df, unsigned int passbit, unsigned int* seg, double& fx,
-                                   double& fy, double& fz, double& vir, double& pot) {
+                                   double& fy, double& fz, double& vir, double& pot,
+                                   int di0 = -NR, int di1 = NR) {
   static_assert(DEPTH == 0 || DEPTH >= CH, "a chunk must fit the deferral buffer");
   constexpr int W = 2 * NR + 1;
   const double L = R.side;
@@ -981,7 +985,7 @@ __device__ inline void scan_chunks(const RepDev& R, int n, double cs, double sig
     g2hi[k] = hi * hi;
   }
   const double selfw = self >= 0 ? (double)self : -1.0;
-  for (int di = -NR; di <= NR; di++) {
+  for (int di = di0; di <= di1; di++) {
     const double gx = slab_gap(di, cs, ux);
     const double gx2 = gx * gx;
     if (gx2 > lim) continue;
@@ -1228,13 +1232,15 @@ __global__ void __launch_bounds__(kBlk)
 
 // One stencil pass (0: same species, 1: the other species, main.cu:576-585)
 // of replica-local particle i with a slab walk (WALK 1..3), on grids of at
-// least 2 nrange + 3 cells.  Returns false when the particle has no cell in
-// that grid.  REC: record the partners in df only (see scan_chunks).
+// least 2 nrange + 3 cells.  part selects the slabs di: -1 all, 0 = [-nrange,
+// 0], 1 = [1, nrange].  Returns false when the particle has no cell in that
+// grid.
+// REC: record the partners in df only (see scan_chunks).
 template <int POT, int DEPTH, int WALK, bool REC>
 __device__ inline bool walk_pass(const RepDev& R, int i, int pass, double x, double y, double z,
                                  const int* __restrict__ rs, const double4* __restrict__ cpos,
                                  Defer& df, unsigned int* seg, double& fx, double& fy,
-                                 double& fz, double& v, double& u) {
+                                 double& fz, double& v, double& u, int part = -1) {
   const int g = i < R.nb ? 0 : 1;
   const int gg = pass == 0 ? g : 1 - g;
   const int nr = pass == 0 ? R.nrange_same[g] : R.nrange_cross[g];
@@ -1248,23 +1254,64 @@ __device__ inline bool walk_pass(const RepDev& R, int i, int pass, double x, dou
   const int* rs_g = rs + R.rs_off[gg];
   const double4* cp_g = cpos + R.tag_off[gg];
   const unsigned int pb = pass == 0 ? 0u : kPassBit;
+  const int di0 = part == 1 ? 1 : -nr;
+  const int di1 = part == 0 ? 0 : nr;
   if (WALK == 1) {
     if (nr == 1)
       scan_slabs<POT, 1, DEPTH, REC>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df, pb,
-                                     seg, fx, fy, fz, v, u);
+                                     seg, fx, fy, fz, v, u, di0, di1);
     else
       scan_slabs<POT, 2, DEPTH, REC>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df, pb,
-                                     seg, fx, fy, fz, v, u);
+                                     seg, fx, fy, fz, v, u, di0, di1);
   } else {
     constexpr int CH = WALK == 2 ? 2 : 4;
     if (nr == 1)
       scan_chunks<POT, 1, DEPTH, CH, REC>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g,
-                                          df, pb, seg, fx, fy, fz, v, u);
+                                          df, pb, seg, fx, fy, fz, v, u, di0, di1);
     else
       scan_chunks<POT, 2, DEPTH, CH, REC>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g,
-                                          df, pb, seg, fx, fy, fz, v, u);
+                                          df, pb, seg, fx, fy, fz, v, u, di0, di1);
   }
   return true;
+}
+
+// The five terms one recorded pair adds (pair_add's operations, split
+// from the accumulation): force components, virial term, energy term.
+template <int POT>
+__device__ inline void pair_terms(const RepDev& R, double x, double y, double z, const Defer& d,
+                                  unsigned int wc, double& qx, double& qy, double& qz,
+                                  double& wv, double& e) {
+  const double L = R.side;
+  const double4 qc = ((wc & kPassBit) ? d.cp1 : d.cp0)[wc & kSlotMask];
+  const unsigned int cx = (wc >> kSlotBits) & 3u, cy = (wc >> (kSlotBits + 2)) & 3u,
+                     cz = (wc >> (kSlotBits + 4)) & 3u;
+  const double dx = (qc.x - x) + (cx == 0u ? 0.0 : (cx == 1u ? -L : L));
+  const double dy = (qc.y - y) + (cy == 0u ? 0.0 : (cy == 1u ? -L : L));
+  const double dz = (qc.z - z) + (cz == 0u ? 0.0 : (cz == 1u ? -L : L));
+  const double dist2 = dx * dx + dy * dy + dz * dz;
+  const double dist_inv = sqrt(1.0 / dist2);
+  double nf;
+  bd::pair_energy<POT>(R.kappa, (wc & kPassBit) ? d.sigma1 : d.sigma0, dist_inv, e, nf);
+  qx = nf * dist_inv * dx;
+  qy = nf * dist_inv * dy;
+  qz = nf * dist_inv * dz;
+  wv = 0.5 * (dx * qx + dy * qy + dz * qz);
+}
+
+// Pair terms of the records e0 .. e1-1 of d's column, added in order.
+template <int POT>
+__device__ inline void drain_range(const RepDev& R, double x, double y, double z, const Defer& d,
+                                   int e0, int e1, double& fx, double& fy, double& fz,
+                                   double& vir, double& pot) {
+  for (int e = e0; e < e1; e++) {
+    double qx, qy, qz, wv, en;
+    pair_terms<POT>(R, x, y, z, d, d.buf[e * kBlk], qx, qy, qz, wv, en);
+    pot = pot + en;
+    fx += qx;
+    fy += qy;
+    fz += qz;
+    vir -= wv;
+  }
 }
 
 // Stencil passes split over two wavefronts (no reference counterpart): for
@@ -1338,6 +1385,119 @@ __global__ void __launch_bounds__(2 * kBlk)
         drain<POT>(R, x, y, z, d1, fx, fy, fz, v, u);
       } else if (walk_pass<POT, DEPTH, WALK, false>(R, i, 1, x, y, z, rs, cpos, df, seg, fx, fy,
                                                     fz, v, u)) {
+        drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
+      }
+    }
+  }
+  ffx[p] = fx;
+  ffy[p] = fy;
+  ffz[p] = fz;
+  vir[p] = v;
+  pot[p] = u;
+}
+
+// Stencil passes split over four wavefronts (no reference counterpart): for
+// the same 64 particles, wavefront w walks slab part (w & 1) of pass (w >> 1)
+// (walk_pass), so the walk of one particle runs on four lanes at once.
+// Wavefronts 1..3 record their partners and evaluate the terms of the first
+// kTermCap of them into LDS; after the barrier wavefront 0 adds, in walking
+// order, its own terms, then each other wavefront's stored terms and the
+// terms of its records beyond kTermCap.  A part whose partners overflow the
+// DEPTH-entry column is walked again by wavefront 0.  Bit-identical to the
+// single-wavefront kernel.  Wavefront 0 takes the part with the centre slab
+// of the same-species pass, whose terms it adds as it goes.
+constexpr int kTermCap = 5;
+
+template <int POT, int DEPTH, int WALK>
+__global__ void __launch_bounds__(4 * kBlk)
+    k_forces_quad(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+                  const double* __restrict__ px, const double* __restrict__ py,
+                  const double* __restrict__ pz, const int* __restrict__ rs,
+                  const double4* __restrict__ cpos, double* __restrict__ ffx,
+                  double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
+                  double* __restrict__ pot) {
+  static_assert(DEPTH >= kTermCap && WALK >= 1, "the split needs a slab walk with deferral");
+  __shared__ unsigned int dbuf[4 * DEPTH * kBlk];
+  __shared__ unsigned int sbuf[4 * 2 * kSlabSegs * kBlk];
+  __shared__ double terms[3 * kTermCap * 5 * kBlk];
+  __shared__ int nrec[3 * kBlk];
+  const int4 t = tbl[blockIdx.x];
+  if (t.x < 0) return;
+  const int lane = (int)threadIdx.x % kBlk;
+  const int w = (int)threadIdx.x / kBlk;
+  const int i = t.y + lane;
+  const bool live = i < t.z;
+  const RepDev& R = reps[t.x];
+  const int p = R.p_off + i;
+  const int g = i < R.nb ? 0 : 1;
+  double x = 0.0, y = 0.0, z = 0.0;
+  if (live) {
+    x = px[p];
+    y = py[p];
+    z = pz[p];
+  }
+  double fx = 0.0, fy = 0.0, fz = 0.0, v = 0.0, u = 0.0;
+  Defer df{cpos + R.tag_off[g], cpos + R.tag_off[1 - g], R.sigma[g], R.sigma_bs,
+           dbuf + w * DEPTH * kBlk + lane, 0};
+  unsigned int* seg = sbuf + w * 2 * kSlabSegs * kBlk + lane;
+  bool ok = true;
+  if (w > 0) {
+    int nw = -1;  // -1: wavefront 0 walks the part itself; -2: no cell in that grid
+    if (live && R.fast) {
+      if (walk_pass<POT, DEPTH, WALK, true>(R, i, w >> 1, x, y, z, rs, cpos, df, seg, fx, fy,
+                                            fz, v, u, w & 1)) {
+        nw = df.nd <= DEPTH ? df.nd : -1;
+        double* tq = terms + (w - 1) * kTermCap * 5 * kBlk + lane;
+        const int m = nw < kTermCap ? nw : kTermCap;
+        for (int e = 0; e < m; e++) {
+          double qx, qy, qz, wv, en;
+          pair_terms<POT>(R, x, y, z, df, df.buf[e * kBlk], qx, qy, qz, wv, en);
+          tq[(5 * e + 0) * kBlk] = qx;
+          tq[(5 * e + 1) * kBlk] = qy;
+          tq[(5 * e + 2) * kBlk] = qz;
+          tq[(5 * e + 3) * kBlk] = wv;
+          tq[(5 * e + 4) * kBlk] = en;
+        }
+      } else {
+        nw = -2;
+      }
+    }
+    nrec[(w - 1) * kBlk + lane] = nw;
+  } else if (live) {
+    if (R.fast) {
+      ok = walk_pass<POT, DEPTH, WALK, false>(R, i, 0, x, y, z, rs, cpos, df, seg, fx, fy, fz, v,
+                                              u, 0);
+      if (ok) drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
+    } else {
+      forces_global<POT, false>(R, i, x, y, z, rs, cpos, fx, fy, fz, v, u);
+    }
+  }
+  __syncthreads();
+  if (w != 0 || !live) return;
+  const int n1 = nrec[lane], n2 = nrec[kBlk + lane], n3 = nrec[2 * kBlk + lane];
+  // a particle without a cell in a grid (non-finite position) skips the rest
+  if (R.fast && ok && n1 != -2 && n2 != -2 && n3 != -2) {
+    const int nws[3] = {n1, n2, n3};
+#pragma unroll
+    for (int k = 1; k < 4; k++) {
+      const int nw = nws[k - 1];
+      if (nw >= 0) {
+        const double* tq = terms + (k - 1) * kTermCap * 5 * kBlk + lane;
+        const int m = nw < kTermCap ? nw : kTermCap;
+        for (int e = 0; e < m; e++) {
+          u = u + tq[(5 * e + 4) * kBlk];
+          fx += tq[(5 * e + 0) * kBlk];
+          fy += tq[(5 * e + 1) * kBlk];
+          fz += tq[(5 * e + 2) * kBlk];
+          v -= tq[(5 * e + 3) * kBlk];
+        }
+        if (nw > kTermCap) {
+          const Defer dk{df.cp0, df.cp1, df.sigma0, df.sigma1, dbuf + k * DEPTH * kBlk + lane,
+                         nw};
+          drain_range<POT>(R, x, y, z, dk, kTermCap, nw, fx, fy, fz, v, u);
+        }
+      } else if (walk_pass<POT, DEPTH, WALK, false>(R, i, k >> 1, x, y, z, rs, cpos, df, seg, fx,
+                                                    fy, fz, v, u, k & 1)) {
         drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
       }
     }
@@ -1942,7 +2102,8 @@ struct bd_sys {
   // walk of the default force kernel: 0 rows, 1 slabs, 2 / 3 chunked slabs
   int walk = 1;
   // stencil passes of the thread-per-particle kernel: 0 one wavefront, 1 the
-  // two passes on two wavefronts (slab walks with deferral only)
+  // two passes on two wavefronts, 2 the two halves of each pass on four
+  // wavefronts (slab walks with deferral only)
   int pass_split = 0;
   bool defer_ok = true;  // every grid's slot index fits the 26-bit record
   std::vector<int4> tbl_split_h;
@@ -2027,12 +2188,39 @@ void launch_pair_k(bd_sys* s, const int4* tbl, int nblk) {
   }
 }
 
+template <int DEPTH, int WALK>
+void launch_quad_k(bd_sys* s, const int4* tbl, int nblk) {
+  switch (s->pot_kind) {
+    case BD_POT_WCA:
+      k_forces_quad<BD_POT_WCA, DEPTH, WALK><<<nblk, 4 * kBlk, 0, s->stream>>>(
+          s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
+          s->pot);
+      break;
+    case BD_POT_WCA_MODIFIED:
+      k_forces_quad<BD_POT_WCA_MODIFIED, DEPTH, WALK><<<nblk, 4 * kBlk, 0, s->stream>>>(
+          s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
+          s->pot);
+      break;
+    default:
+      k_forces_quad<BD_POT_HARMONIC, DEPTH, WALK><<<nblk, 4 * kBlk, 0, s->stream>>>(
+          s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
+          s->pot);
+      break;
+  }
+}
+
 template <int WALK>
 void launch_pair_w(bd_sys* s, const int4* tbl, int nblk) {
-  if (s->defer == 16)
+  if (s->pass_split == 2) {
+    if (s->defer == 16)
+      launch_quad_k<16, WALK>(s, tbl, nblk);
+    else
+      launch_quad_k<32, WALK>(s, tbl, nblk);
+  } else if (s->defer == 16) {
     launch_pair_k<16, WALK>(s, tbl, nblk);
-  else
+  } else {
     launch_pair_k<32, WALK>(s, tbl, nblk);
+  }
 }
 
 template <bool NEIGH>
@@ -2214,7 +2402,8 @@ int check_flags(bd_sys* s) {
   HIP_TRY(hipStreamSynchronize(s->stream));
   for (int r = 0; r < s->R; r++) {
     if (f[r] & kFlagNonFinite)
-      return fail(BD_E_NAN, "replica " + std::to_string(r) + ": non-finite position");
+      return fail(BD_E_NAN, "replica " + std::to_string(r) +
+                                ": non-finite position, or one outside the cell grids");
     if (f[r] & kFlagBrick)
       return fail(BD_E_STATE, "replica " + std::to_string(r) + ": brick exceeds its LDS table");
     if (f[r] & kFlagOverflow)
@@ -3009,6 +3198,8 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
       {0, 0, BD_WALK_CHUNK4, 0},  {0, 16, BD_WALK_CHUNK4, 0}, {0, 32, BD_WALK_CHUNK4, 0},
       {0, 16, BD_WALK_SLABS, 1},  {0, 32, BD_WALK_SLABS, 1},  {0, 16, BD_WALK_CHUNK2, 1},
       {0, 32, BD_WALK_CHUNK2, 1}, {0, 16, BD_WALK_CHUNK4, 1}, {0, 32, BD_WALK_CHUNK4, 1},
+      {0, 16, BD_WALK_SLABS, 2},  {0, 32, BD_WALK_SLABS, 2},  {0, 16, BD_WALK_CHUNK2, 2},
+      {0, 32, BD_WALK_CHUNK2, 2}, {0, 16, BD_WALK_CHUNK4, 2}, {0, 32, BD_WALK_CHUNK4, 2},
       {1, 0, BD_WALK_ROWS, 0},    {2, 0, BD_WALK_ROWS, 0},    {3, 0, BD_WALK_ROWS, 0},
       {4, 0, BD_WALK_ROWS, 0}};
   Shape best{0, 0, BD_WALK_ROWS, 0};
@@ -3085,7 +3276,7 @@ int bd_get_walk(const bd_sys* s, int32_t* walk) {
 
 int bd_set_pass_split(bd_sys* s, int32_t split) {
   if (!s) return fail(BD_E_ARG, "null handle");
-  if (split != 0 && split != 1) return fail(BD_E_ARG, "pass split must be 0 or 1");
+  if (split < 0 || split > 2) return fail(BD_E_ARG, "pass split must be 0, 1 or 2");
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (s->graph) {
     (void)hipGraphExecDestroy(s->graph);

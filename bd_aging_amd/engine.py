@@ -216,8 +216,8 @@ class Engine:
         return v.value
 
     def set_pass_split(self, split):
-        """1: the cross-species stencil pass on a second wavefront (slab walks
-        with deferral); 0: one wavefront per particle."""
+        """Wavefronts per particle block (slab walks with deferral): 0 one,
+        1 two (one per stencil pass), 2 four (half a pass each)."""
         check(self.lib.bd_set_pass_split(self.h, int(split)))
 
     def pass_split(self):

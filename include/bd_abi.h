@@ -42,7 +42,8 @@ enum {
   BD_E_ARG = 1,           /* invalid argument or geometry */
   BD_E_HIP = 2,           /* HIP runtime error */
   BD_E_CELL_OVERFLOW = 3, /* a cell holds >= ntags particles (main.cu:655-659) */
-  BD_E_NAN = 4,           /* a non-finite position appeared */
+  BD_E_NAN = 4,           /* a non-finite position appeared (or one more than a
+                             cell side outside the box) */
   BD_E_STATE = 5,         /* call out of order */
   BD_E_NOMEM = 6
 };
@@ -182,10 +183,13 @@ int bd_get_walk(const bd_sys* s, int32_t* walk);
 /* Stencil passes over wavefronts (no reference counterpart): 0 (default)
  * walks both passes of a particle (main.cu:576-585) in one lane; 1 walks the
  * cross-species pass on a second wavefront at the same time, which only
- * records its partners for the first wavefront to evaluate after its own
- * (twice the wavefronts, for systems too small to fill the GPU).  Takes
- * effect with a slab walk (BD_WALK_SLABS, _CHUNK2, _CHUNK4) at a nonzero
- * deferral depth.  Bit-identical.  BD_E_ARG for values other than 0, 1. */
+ * records its partners for the first wavefront to evaluate after its own;
+ * 2 walks the two halves of each pass on four wavefronts, the other three
+ * evaluating the terms of their first partners into LDS for the first to
+ * add in order.  More wavefronts for the same work, for systems too small to
+ * fill the GPU.  Takes effect with a slab walk (BD_WALK_SLABS, _CHUNK2,
+ * _CHUNK4) at a nonzero deferral depth.  Bit-identical.  BD_E_ARG for values
+ * other than 0, 1, 2. */
 int bd_set_pass_split(bd_sys* s, int32_t split);
 int bd_get_pass_split(const bd_sys* s, int32_t* split);
 

@@ -143,7 +143,8 @@ def test_every_brick_shape_bit_identical(config1_positions, pot):
             for depth in ((0, 16, 32) if b == 0 else (16,)):
                 e.set_defer_depth(depth)
                 assert e.defer_depth() == depth
-                for split in ((0, 1) if b == 0 and walk != abi.BD_WALK_ROWS else (0,)):
+                for split in ((0, 1, 2) if b == 0 and walk != abi.BD_WALK_ROWS and depth
+                              else (0,)):
                     e.set_pass_split(split)
                     assert e.pass_split() == split
                     e.compute_forces()
@@ -159,7 +160,7 @@ def test_every_brick_shape_bit_identical(config1_positions, pot):
     for bad in (-1, 4):
         with pytest.raises(BDError):
             e.set_walk(bad)
-    for bad in (-1, 2):
+    for bad in (-1, 3):
         with pytest.raises(BDError):
             e.set_pass_split(bad)
     e.close()
@@ -189,13 +190,14 @@ def test_pass_split_column_overflow(config1_positions):
     e.cell_locate()
     for walk in (abi.BD_WALK_SLABS, abi.BD_WALK_CHUNK2, abi.BD_WALK_CHUNK4):
         for depth in (16, 32):
-            e.set_walk(walk)
-            e.set_defer_depth(depth)
-            e.set_pass_split(1)
-            e.compute_forces()
-            fe, ve, ue = e.forces()
-            assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and \
-                np.array_equal(ue, uo), (walk, depth)
+            for split in (1, 2):
+                e.set_walk(walk)
+                e.set_defer_depth(depth)
+                e.set_pass_split(split)
+                e.compute_forces()
+                fe, ve, ue = e.forces()
+                assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and \
+                    np.array_equal(ue, uo), (walk, depth, split)
     e.close()
     o.close()
 
@@ -222,7 +224,9 @@ def test_forces_bit_exact_at_bench_size(plain_positions):
     shapes = [(0, 0, 0, 0), (0, 16, 0, 0), (0, 32, 0, 0), (0, 0, 1, 0), (0, 16, 1, 0),
               (0, 32, 1, 0), (0, 0, 2, 0), (0, 16, 2, 0), (0, 32, 2, 0), (0, 0, 3, 0),
               (0, 16, 3, 0), (0, 32, 3, 0), (0, 16, 1, 1), (0, 32, 1, 1), (0, 16, 2, 1),
-              (0, 32, 2, 1), (0, 16, 3, 1), (0, 32, 3, 1), (1, 16, 0, 0), (2, 16, 0, 0)]
+              (0, 32, 2, 1), (0, 16, 3, 1), (0, 32, 3, 1), (0, 16, 1, 2), (0, 32, 1, 2),
+              (0, 16, 2, 2), (0, 32, 2, 2), (0, 16, 3, 2), (0, 32, 3, 2), (1, 16, 0, 0),
+              (2, 16, 0, 0)]
     for b, depth, walk, split in shapes:
         e.set_brick(b)
         e.set_defer_depth(depth)
