@@ -102,11 +102,14 @@ SIGNATURES = {
     "bd_get_walk": (C.c_int, [_P, _I]),
     "bd_set_pass_split": (C.c_int, [_P, C.c_int32]),
     "bd_get_pass_split": (C.c_int, [_P, _I]),
+    "bd_set_fused_update": (C.c_int, [_P, C.c_int32]),
+    "bd_get_fused_update": (C.c_int, [_P, _I]),
     "bd_set_defer_depth": (C.c_int, [_P, C.c_int32]),
     "bd_get_defer_depth": (C.c_int, [_P, _I]),
     "bd_set_brick": (C.c_int, [_P, C.c_int32]),
     "bd_get_brick": (C.c_int, [_P, _I]),
     "bd_kernel_times": (C.c_int, [_P, C.c_int32, _D]),
+    "bd_kernel_times_fused": (C.c_int, [_P, C.c_int32, _D]),
     "bd_checkpoint_save": (C.c_int, [_P, C.c_char_p]),
     "bd_checkpoint_load": (C.c_int, [_P, C.c_char_p]),
     "bd_get_rng_lane": (C.c_int, [_P, C.c_int32, C.c_int32, _U, _U, _U]),

@@ -204,6 +204,56 @@ __global__ void __launch_bounds__(kBlk)
   insert_cell(R, t.x, i, px[p], py[p], pz[p], nocup, ins, flags);
 }
 
+// Four rannew64 draws of one lane (rannew64.cu:99-126): ring slot np and
+// its lag partner, the LCG; np steps down the ring.
+__device__ inline void draw4(uint64_t* rl, size_t stride, int& np, uint64_t& lcg,
+                             uint64_t d[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int nq = np >= bd::kLagGap ? np - bd::kLagGap : np + (bd::kRingLen - bd::kLagGap);
+    const uint64_t x = rl[(size_t)np * stride] - rl[(size_t)nq * stride];
+    rl[(size_t)np * stride] = x;
+    lcg = bd::kLcgA * lcg + bd::kLcgC;
+    d[j] = x - lcg;
+    np = np == 0 ? bd::kRingLen - 1 : np - 1;
+  }
+}
+
+// Ermak-McCammon move of replica-local particle i (update_ermak.cu:91-160)
+// from (x, y, z) with force (fx, fy, fz) and normals (xi, yi, zi) at
+// temperature T: raw and wrapped positions written, the wrapped one inserted
+// in its cell (do_bin).  Returns false for a non-finite result.
+__device__ inline bool ermak_move(const RepDev& R, int rep, int i, double T, double x, double y,
+                                  double z, double fx, double fy, double fz, double xi,
+                                  double yi, double zi, double* __restrict__ px,
+                                  double* __restrict__ py, double* __restrict__ pz,
+                                  double* __restrict__ rx, double* __restrict__ ry,
+                                  double* __restrict__ rz, int* __restrict__ nocup,
+                                  int* __restrict__ ins, int* __restrict__ flags, int do_bin) {
+  const int p = R.p_off + i;
+  const double side = R.side, side_inv = R.side_inv, dt = R.dt;
+  const double diameter = i < R.nb ? R.sigma[0] : R.sigma[1];
+  const double dinv = 1.0 / diameter;
+  const double amplitude = sqrt(2.0 * T * dinv * dt);
+  const double dx = fx * dinv * dt + amplitude * xi;
+  const double dy = fy * dinv * dt + amplitude * yi;
+  const double dz = fz * dinv * dt + amplitude * zi;
+  x = x + dx;
+  y = y + dy;
+  z = z + dz;
+  rx[p] = rx[p] + dx;
+  ry[p] = ry[p] + dy;
+  rz[p] = rz[p] + dz;
+  x -= side * floor(side_inv * x);
+  y -= side * floor(side_inv * y);
+  z -= side * floor(side_inv * z);
+  px[p] = x;
+  py[p] = y;
+  pz[p] = z;
+  if (do_bin) insert_cell(R, rep, i, x, y, z, nocup, ins, flags);
+  return isfinite(x) && isfinite(y) && isfinite(z);
+}
+
 // Ermak-McCammon update (update_ermak.cu:91-160) with rannew64 lanes.
 // One thread per lane; lane l serves particles l, l+L, l+2L ... and takes
 // four draws per served slot, so every lane advances 4*ceil(N/L) draws per
@@ -226,7 +276,6 @@ __device__ __forceinline__ void k_update_bin_block(
   const RepDev& R = reps[t.x];
   const int gl = R.lane_off + l;
   const double T = temperature_at(R, *step);
-  const double side = R.side, side_inv = R.side_inv, dt = R.dt;
   uint64_t lcg = lcgs[gl];
   int np = bd::kRingLen - 1 - (int)(draws[t.x] % (uint64_t)bd::kRingLen);
   uint64_t* rl = ring + gl;
@@ -234,41 +283,34 @@ __device__ __forceinline__ void k_update_bin_block(
   bool bad = false;
   for (int k = 0; k < R.per_lane; k++) {
     uint64_t d[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int nq = np >= bd::kLagGap ? np - bd::kLagGap : np + (bd::kRingLen - bd::kLagGap);
-      const uint64_t x = rl[(size_t)np * stride] - rl[(size_t)nq * stride];
-      rl[(size_t)np * stride] = x;
-      lcg = bd::kLcgA * lcg + bd::kLcgC;
-      d[j] = x - lcg;
-      np = np == 0 ? bd::kRingLen - 1 : np - 1;
-    }
+    draw4(rl, stride, np, lcg, d);
     const int i = l + k * R.lanes;
     if (i >= R.n) continue;
     const int p = R.p_off + i;
-    const double diameter = i < R.nb ? R.sigma[0] : R.sigma[1];
-    const double dinv = 1.0 / diameter;
-    const double amplitude = sqrt(2.0 * T * dinv * dt);
     double xi, yi, zi;
     bd::gauss3(d[0], d[1], d[2], d[3], xi, yi, zi);
-    const double dx = fx[p] * dinv * dt + amplitude * xi;
-    const double dy = fy[p] * dinv * dt + amplitude * yi;
-    const double dz = fz[p] * dinv * dt + amplitude * zi;
-    double x = px[p] + dx, y = py[p] + dy, z = pz[p] + dz;
-    rx[p] = rx[p] + dx;
-    ry[p] = ry[p] + dy;
-    rz[p] = rz[p] + dz;
-    x -= side * floor(side_inv * x);
-    y -= side * floor(side_inv * y);
-    z -= side * floor(side_inv * z);
-    px[p] = x;
-    py[p] = y;
-    pz[p] = z;
-    if (!(isfinite(x) && isfinite(y) && isfinite(z))) bad = true;
-    if (do_bin) insert_cell(R, t.x, i, x, y, z, nocup, ins, flags);
+    if (!ermak_move(R, t.x, i, T, px[p], py[p], pz[p], fx[p], fy[p], fz[p], xi, yi, zi, px, py,
+                    pz, rx, ry, rz, nocup, ins, flags, do_bin))
+      bad = true;
   }
   lcgs[gl] = lcg;
   if (bad) atomicOr(&flags[t.x], kFlagNonFinite);
+}
+
+// The normals of particle i's next move, for a replica whose lanes are its
+// particles (one draw block per lane and step): the same draws as
+// k_update_bin_block, taken by the force kernel ahead of the move.
+__device__ inline void lane_normals(const RepDev& R, int rep, int i, uint64_t* __restrict__ ring,
+                                    uint64_t* __restrict__ lcgs,
+                                    const uint64_t* __restrict__ draws, int total_lanes,
+                                    double& xi, double& yi, double& zi) {
+  const int gl = R.lane_off + i;
+  uint64_t lcg = lcgs[gl];
+  int np = bd::kRingLen - 1 - (int)(draws[rep] % (uint64_t)bd::kRingLen);
+  uint64_t d[4];
+  draw4(ring + gl, (size_t)total_lanes, np, lcg, d);
+  lcgs[gl] = lcg;
+  bd::gauss3(d[0], d[1], d[2], d[3], xi, yi, zi);
 }
 
 __global__ void __launch_bounds__(kBlk)
@@ -1178,15 +1220,39 @@ __device__ inline void forces_global(const RepDev& R, int i, double x, double y,
 // NEIGH records the interacting partners in traversal order (test hook).
 // DEPTH: per-thread LDS entries for deferred pair terms (0 = direct);
 // WALK: 0 row walk, 1 slab walk, 2 / 3 chunked slab walk (2 / 4 candidates per load round).
-template <int POT, bool NEIGH, int DEPTH, int WALK>
+// What a force kernel needs to also move its particles (the update of the
+// next step, fused): rannew64 rings and LCGs, draw counters, the step index
+// (temperature), raw positions and the binning arrays.
+struct MoveArgs {
+  uint64_t* ring;
+  uint64_t* lcgs;
+  const uint64_t* draws;
+  const int64_t* step;
+  int total_lanes;
+  double *rx, *ry, *rz;
+  int *nocup, *ins, *flags;
+};
+
+// The fused move of particle i once its force is known (replicas with one
+// particle per lane only).
+__device__ inline void fused_move(const RepDev& R, int rep, int i, double x, double y, double z,
+                                  double fx, double fy, double fz, double xi, double yi,
+                                  double zi, double* px, double* py, double* pz,
+                                  const MoveArgs& mv) {
+  const double T = temperature_at(R, *mv.step);
+  if (!ermak_move(R, rep, i, T, x, y, z, fx, fy, fz, xi, yi, zi, px, py, pz, mv.rx, mv.ry,
+                  mv.rz, mv.nocup, mv.ins, mv.flags, 1))
+    atomicOr(&mv.flags[rep], kFlagNonFinite);
+}
+
+template <int POT, bool NEIGH, int DEPTH, int WALK, bool UPD = false>
 __device__ __forceinline__ void k_forces_block(
     int vb, const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
-             const double* __restrict__ px, const double* __restrict__ py,
-             const double* __restrict__ pz, const int* __restrict__ rs,
-             const double4* __restrict__ cpos, double* __restrict__ ffx,
-             double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
-             double* __restrict__ pot, int* __restrict__ ncounts, int* __restrict__ nlist,
-             int maxn) {
+             double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+             const int* __restrict__ rs, const double4* __restrict__ cpos,
+             double* __restrict__ ffx, double* __restrict__ ffy, double* __restrict__ ffz,
+             double* __restrict__ vir, double* __restrict__ pot, int* __restrict__ ncounts,
+             int* __restrict__ nlist, int maxn, const MoveArgs* mv = nullptr) {
   __shared__ unsigned int dbuf[DEPTH > 0 ? DEPTH * kBlk : 1];
   __shared__ unsigned int sbuf[WALK >= 1 ? 2 * kSlabSegs * kBlk : 1];
   const int4 t = tbl[vb];
@@ -1215,19 +1281,31 @@ __device__ __forceinline__ void k_forces_block(
   ffz[p] = fz;
   vir[p] = v;
   pot[p] = u;
+  if (UPD && !NEIGH) {
+    // the walks read the packed copy (cpos), never px/py/pz, so the move can
+    // overwrite this particle's position while other lanes still walk
+    double xi, yi, zi;
+    lane_normals(R, t.x, i, mv->ring, mv->lcgs, mv->draws, mv->total_lanes, xi, yi, zi);
+    fused_move(R, t.x, i, x, y, z, fx, fy, fz, xi, yi, zi, px, py, pz, *mv);
+  }
 }
 
-template <int POT, bool NEIGH, int DEPTH, int WALK>
-__global__ void __launch_bounds__(kBlk)
+// Forces of the current positions; with UPD also the next step's update of
+// every particle from those forces (the binning into the insertion lists
+// included), which k_update_bin would otherwise do in a launch of its own.
+// The fused slab walk at depth <= 16 is held to the unfused kernel's four
+// wavefronts per SIMD (the move alone would take it two registers past 128).
+template <int POT, bool NEIGH, int DEPTH, int WALK, bool UPD>
+__global__ void __launch_bounds__(kBlk, (UPD && WALK == 1 && DEPTH <= 16) ? 4 : 1)
     k_forces(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
-             const double* __restrict__ px, const double* __restrict__ py,
-             const double* __restrict__ pz, const int* __restrict__ rs,
-             const double4* __restrict__ cpos, double* __restrict__ ffx,
-             double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
-             double* __restrict__ pot, int* __restrict__ ncounts, int* __restrict__ nlist,
-             int maxn) {
-  k_forces_block<POT, NEIGH, DEPTH, WALK>((int)blockIdx.x, reps, tbl, px, py, pz, rs, cpos, ffx,
-                                          ffy, ffz, vir, pot, ncounts, nlist, maxn);
+             double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+             const int* __restrict__ rs, const double4* __restrict__ cpos,
+             double* __restrict__ ffx, double* __restrict__ ffy, double* __restrict__ ffz,
+             double* __restrict__ vir, double* __restrict__ pot, int* __restrict__ ncounts,
+             int* __restrict__ nlist, int maxn, MoveArgs mv) {
+  k_forces_block<POT, NEIGH, DEPTH, WALK, UPD>((int)blockIdx.x, reps, tbl, px, py, pz, rs, cpos,
+                                               ffx, ffy, ffz, vir, pot, ncounts, nlist, maxn,
+                                               &mv);
 }
 
 // One stencil pass (0: same species, 1: the other species, main.cu:576-585)
@@ -1323,18 +1401,18 @@ __device__ inline void drain_range(const RepDev& R, double x, double y, double z
 // cross-species partners overflow the DEPTH-entry column has that pass walked
 // again by wavefront 0.  Twice the wavefronts of k_forces for the same work,
 // for systems too small to fill the GPU with one lane per particle.
-template <int POT, int DEPTH, int WALK>
+template <int POT, int DEPTH, int WALK, bool UPD>
 __global__ void __launch_bounds__(2 * kBlk)
     k_forces_pair(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
-                  const double* __restrict__ px, const double* __restrict__ py,
-                  const double* __restrict__ pz, const int* __restrict__ rs,
-                  const double4* __restrict__ cpos, double* __restrict__ ffx,
-                  double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
-                  double* __restrict__ pot) {
+                  double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+                  const int* __restrict__ rs, const double4* __restrict__ cpos,
+                  double* __restrict__ ffx, double* __restrict__ ffy, double* __restrict__ ffz,
+                  double* __restrict__ vir, double* __restrict__ pot, MoveArgs mv) {
   static_assert(DEPTH > 0 && WALK >= 1, "the split needs a slab walk with deferral");
   __shared__ unsigned int dbuf[2 * DEPTH * kBlk];
   __shared__ unsigned int sbuf[2 * 2 * kSlabSegs * kBlk];
   __shared__ int n1s[kBlk];
+  __shared__ double nrm[UPD ? 3 * kBlk : 1];
   const int4 t = tbl[blockIdx.x];
   if (t.x < 0) return;
   const int lane = (int)threadIdx.x % kBlk;
@@ -1365,6 +1443,10 @@ __global__ void __launch_bounds__(2 * kBlk)
         n1 = -2;
     }
     n1s[lane] = n1;
+    if (UPD && live) {  // the next move's normals, while wavefront 0 still walks
+      lane_normals(R, t.x, i, mv.ring, mv.lcgs, mv.draws, mv.total_lanes, nrm[lane],
+                   nrm[kBlk + lane], nrm[2 * kBlk + lane]);
+    }
   } else if (live) {
     if (R.fast)
       ok = walk_pass<POT, DEPTH, WALK, false>(R, i, 0, x, y, z, rs, cpos, df, seg, fx, fy, fz, v,
@@ -1394,6 +1476,9 @@ __global__ void __launch_bounds__(2 * kBlk)
   ffz[p] = fz;
   vir[p] = v;
   pot[p] = u;
+  if (UPD)
+    fused_move(R, t.x, i, x, y, z, fx, fy, fz, nrm[lane], nrm[kBlk + lane], nrm[2 * kBlk + lane],
+               px, py, pz, mv);
 }
 
 // Stencil passes split over four wavefronts (no reference counterpart): for
@@ -1408,19 +1493,19 @@ __global__ void __launch_bounds__(2 * kBlk)
 // of the same-species pass, whose terms it adds as it goes.
 constexpr int kTermCap = 5;
 
-template <int POT, int DEPTH, int WALK>
+template <int POT, int DEPTH, int WALK, bool UPD>
 __global__ void __launch_bounds__(4 * kBlk)
     k_forces_quad(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
-                  const double* __restrict__ px, const double* __restrict__ py,
-                  const double* __restrict__ pz, const int* __restrict__ rs,
-                  const double4* __restrict__ cpos, double* __restrict__ ffx,
-                  double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
-                  double* __restrict__ pot) {
+                  double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+                  const int* __restrict__ rs, const double4* __restrict__ cpos,
+                  double* __restrict__ ffx, double* __restrict__ ffy, double* __restrict__ ffz,
+                  double* __restrict__ vir, double* __restrict__ pot, MoveArgs mv) {
   static_assert(DEPTH >= kTermCap && WALK >= 1, "the split needs a slab walk with deferral");
   __shared__ unsigned int dbuf[4 * DEPTH * kBlk];
   __shared__ unsigned int sbuf[4 * 2 * kSlabSegs * kBlk];
   __shared__ double terms[3 * kTermCap * 5 * kBlk];
   __shared__ int nrec[3 * kBlk];
+  __shared__ double nrm[UPD ? 3 * kBlk : 1];
   const int4 t = tbl[blockIdx.x];
   if (t.x < 0) return;
   const int lane = (int)threadIdx.x % kBlk;
@@ -1463,6 +1548,10 @@ __global__ void __launch_bounds__(4 * kBlk)
       }
     }
     nrec[(w - 1) * kBlk + lane] = nw;
+    if (UPD && w == 3 && live) {  // the next move's normals (the lightest part's wavefront)
+      lane_normals(R, t.x, i, mv.ring, mv.lcgs, mv.draws, mv.total_lanes, nrm[lane],
+                   nrm[kBlk + lane], nrm[2 * kBlk + lane]);
+    }
   } else if (live) {
     if (R.fast) {
       ok = walk_pass<POT, DEPTH, WALK, false>(R, i, 0, x, y, z, rs, cpos, df, seg, fx, fy, fz, v,
@@ -1507,6 +1596,9 @@ __global__ void __launch_bounds__(4 * kBlk)
   ffz[p] = fz;
   vir[p] = v;
   pot[p] = u;
+  if (UPD)
+    fused_move(R, t.x, i, x, y, z, fx, fy, fz, nrm[lane], nrm[kBlk + lane], nrm[2 * kBlk + lane],
+               px, py, pz, mv);
 }
 
 // ---------------------------------------------------------------------------
@@ -2105,6 +2197,9 @@ struct bd_sys {
   // two passes on two wavefronts, 2 the two halves of each pass on four
   // wavefronts (slab walks with deferral only)
   int pass_split = 0;
+  // update inside the force kernel: 0 never, 1 wherever the shape allows it,
+  // 2 (auto) with the multi-wavefront pass splits only
+  int fuse = 2;
   bool defer_ok = true;  // every grid's slot index fits the 26-bit record
   std::vector<int4> tbl_split_h;
   int4* tbl_split = nullptr;
@@ -2136,114 +2231,128 @@ void launch_cells(bd_sys* s, int advance) {
                                       s->draws, advance);
 }
 
-template <bool NEIGH, int DEPTH, int WALK>
+MoveArgs move_args(bd_sys* s) {
+  return MoveArgs{s->ring, s->lcg,  s->draws, s->step,  s->lanes_total, s->rx,
+                  s->ry,   s->rz,   s->nocup, s->ins,   s->flags};
+}
+
+// UPD: the force kernel also performs the next step's update (see k_forces).
+template <bool NEIGH, int DEPTH, int WALK, bool UPD>
 void launch_forces_k(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl, int nblk) {
+  const MoveArgs mv = move_args(s);
   switch (s->pot_kind) {
     case BD_POT_WCA:
-      k_forces<BD_POT_WCA, NEIGH, DEPTH, WALK><<<nblk, kBlk, 0, s->stream>>>(
+      k_forces<BD_POT_WCA, NEIGH, DEPTH, WALK, UPD><<<nblk, kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot, ncounts, nlist, maxn);
+          s->pot, ncounts, nlist, maxn, mv);
       break;
     case BD_POT_WCA_MODIFIED:
-      k_forces<BD_POT_WCA_MODIFIED, NEIGH, DEPTH, WALK><<<nblk, kBlk, 0, s->stream>>>(
+      k_forces<BD_POT_WCA_MODIFIED, NEIGH, DEPTH, WALK, UPD><<<nblk, kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot, ncounts, nlist, maxn);
+          s->pot, ncounts, nlist, maxn, mv);
       break;
     default:
-      k_forces<BD_POT_HARMONIC, NEIGH, DEPTH, WALK><<<nblk, kBlk, 0, s->stream>>>(
+      k_forces<BD_POT_HARMONIC, NEIGH, DEPTH, WALK, UPD><<<nblk, kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot, ncounts, nlist, maxn);
+          s->pot, ncounts, nlist, maxn, mv);
       break;
   }
 }
 
-template <int WALK>
+template <int WALK, bool UPD>
 void launch_forces_w(bd_sys* s, const int4* tbl, int nblk) {
   if (s->defer == 0)
-    launch_forces_k<false, 0, WALK>(s, nullptr, nullptr, 0, tbl, nblk);
+    launch_forces_k<false, 0, WALK, UPD>(s, nullptr, nullptr, 0, tbl, nblk);
   else if (s->defer == 16)
-    launch_forces_k<false, 16, WALK>(s, nullptr, nullptr, 0, tbl, nblk);
+    launch_forces_k<false, 16, WALK, UPD>(s, nullptr, nullptr, 0, tbl, nblk);
   else
-    launch_forces_k<false, 32, WALK>(s, nullptr, nullptr, 0, tbl, nblk);
+    launch_forces_k<false, 32, WALK, UPD>(s, nullptr, nullptr, 0, tbl, nblk);
 }
 
-template <int DEPTH, int WALK>
+template <int DEPTH, int WALK, bool UPD>
 void launch_pair_k(bd_sys* s, const int4* tbl, int nblk) {
+  const MoveArgs mv = move_args(s);
   switch (s->pot_kind) {
     case BD_POT_WCA:
-      k_forces_pair<BD_POT_WCA, DEPTH, WALK><<<nblk, 2 * kBlk, 0, s->stream>>>(
+      k_forces_pair<BD_POT_WCA, DEPTH, WALK, UPD><<<nblk, 2 * kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot);
+          s->pot, mv);
       break;
     case BD_POT_WCA_MODIFIED:
-      k_forces_pair<BD_POT_WCA_MODIFIED, DEPTH, WALK><<<nblk, 2 * kBlk, 0, s->stream>>>(
+      k_forces_pair<BD_POT_WCA_MODIFIED, DEPTH, WALK, UPD><<<nblk, 2 * kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot);
+          s->pot, mv);
       break;
     default:
-      k_forces_pair<BD_POT_HARMONIC, DEPTH, WALK><<<nblk, 2 * kBlk, 0, s->stream>>>(
+      k_forces_pair<BD_POT_HARMONIC, DEPTH, WALK, UPD><<<nblk, 2 * kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot);
+          s->pot, mv);
       break;
   }
 }
 
-template <int DEPTH, int WALK>
+template <int DEPTH, int WALK, bool UPD>
 void launch_quad_k(bd_sys* s, const int4* tbl, int nblk) {
+  const MoveArgs mv = move_args(s);
   switch (s->pot_kind) {
     case BD_POT_WCA:
-      k_forces_quad<BD_POT_WCA, DEPTH, WALK><<<nblk, 4 * kBlk, 0, s->stream>>>(
+      k_forces_quad<BD_POT_WCA, DEPTH, WALK, UPD><<<nblk, 4 * kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot);
+          s->pot, mv);
       break;
     case BD_POT_WCA_MODIFIED:
-      k_forces_quad<BD_POT_WCA_MODIFIED, DEPTH, WALK><<<nblk, 4 * kBlk, 0, s->stream>>>(
+      k_forces_quad<BD_POT_WCA_MODIFIED, DEPTH, WALK, UPD><<<nblk, 4 * kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot);
+          s->pot, mv);
       break;
     default:
-      k_forces_quad<BD_POT_HARMONIC, DEPTH, WALK><<<nblk, 4 * kBlk, 0, s->stream>>>(
+      k_forces_quad<BD_POT_HARMONIC, DEPTH, WALK, UPD><<<nblk, 4 * kBlk, 0, s->stream>>>(
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
-          s->pot);
+          s->pot, mv);
       break;
   }
 }
 
-template <int WALK>
+template <int WALK, bool UPD>
 void launch_pair_w(bd_sys* s, const int4* tbl, int nblk) {
   if (s->pass_split == 2) {
     if (s->defer == 16)
-      launch_quad_k<16, WALK>(s, tbl, nblk);
+      launch_quad_k<16, WALK, UPD>(s, tbl, nblk);
     else
-      launch_quad_k<32, WALK>(s, tbl, nblk);
+      launch_quad_k<32, WALK, UPD>(s, tbl, nblk);
   } else if (s->defer == 16) {
-    launch_pair_k<16, WALK>(s, tbl, nblk);
+    launch_pair_k<16, WALK, UPD>(s, tbl, nblk);
   } else {
-    launch_pair_k<32, WALK>(s, tbl, nblk);
+    launch_pair_k<32, WALK, UPD>(s, tbl, nblk);
+  }
+}
+
+template <bool UPD>
+void launch_slab_forces(bd_sys* s, const int4* tbl, int nblk) {
+  if (s->pass_split && s->defer > 0) {
+    if (s->walk == 1)
+      launch_pair_w<1, UPD>(s, tbl, nblk);
+    else if (s->walk == 2)
+      launch_pair_w<2, UPD>(s, tbl, nblk);
+    else
+      launch_pair_w<3, UPD>(s, tbl, nblk);
+  } else if (s->walk == 1) {
+    launch_forces_w<1, UPD>(s, tbl, nblk);
+  } else if (s->walk == 2) {
+    launch_forces_w<2, UPD>(s, tbl, nblk);
+  } else {
+    launch_forces_w<3, UPD>(s, tbl, nblk);
   }
 }
 
 template <bool NEIGH>
 void launch_forces_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl, int nblk) {
-  if (!NEIGH && s->pass_split && s->defer > 0 && s->walk >= 1) {
-    if (s->walk == 1)
-      launch_pair_w<1>(s, tbl, nblk);
-    else if (s->walk == 2)
-      launch_pair_w<2>(s, tbl, nblk);
-    else
-      launch_pair_w<3>(s, tbl, nblk);
-    return;
-  }
   if (NEIGH)
-    launch_forces_k<true, 0, 0>(s, ncounts, nlist, maxn, tbl, nblk);
-  else if (s->walk == 1)
-    launch_forces_w<1>(s, tbl, nblk);
-  else if (s->walk == 2)
-    launch_forces_w<2>(s, tbl, nblk);
-  else if (s->walk == 3)
-    launch_forces_w<3>(s, tbl, nblk);
+    launch_forces_k<true, 0, 0, false>(s, ncounts, nlist, maxn, tbl, nblk);
+  else if (s->walk >= 1)
+    launch_slab_forces<false>(s, tbl, nblk);
   else
-    launch_forces_w<0>(s, tbl, nblk);
+    launch_forces_w<0, false>(s, tbl, nblk);
 }
 
 template <int CAP>
@@ -2288,6 +2397,27 @@ void launch_split_t(bd_sys* s) {
           s->vir, s->pot);
       break;
   }
+}
+
+// Whether the next step's update runs inside the force kernel: a
+// thread-per-particle slab walk in reference order, and one particle per
+// rannew64 lane in every replica.  In auto mode only with a pass split: the
+// split kernels leave the GPU room for the move (plain N=32768: 1.13e9 vs
+// 9.8e8 particle-steps/s), while the single-wavefront walk on a full batch
+// loses more to the added registers than the launch it saves (2.88e9 vs
+// 3.02e9 on 256 replicas).
+bool fusable(const bd_sys* s) {
+  if (s->fuse == 0 || s->order != BD_ORDER_REFERENCE || s->brick != 0 || s->walk < 1)
+    return false;
+  if (s->fuse == 2 && (s->pass_split == 0 || s->defer == 0)) return false;
+  for (int r = 0; r < s->R; r++)
+    if (s->reps[r].per_lane != 1) return false;
+  return true;
+}
+
+// Forces of the current positions and the next step's update in one launch.
+void launch_forces_move(bd_sys* s) {
+  launch_slab_forces<true>(s, s->tbl_part, (int)s->tbl_part_h.size());
 }
 
 void launch_forces(bd_sys* s) {
@@ -2392,6 +2522,24 @@ int setup_bricks(bd_sys* s, int B) {
 void enqueue_step(bd_sys* s) {
   launch_update(s, 1);
   launch_cells(s, 1);
+  launch_forces(s);
+}
+
+// k whole steps.  When fusable, the update of steps 2..k runs inside the
+// force kernel of the step before (same operations on the same values): one
+// launch fewer per step, and the state after the k steps is the same.
+void enqueue_steps(bd_sys* s, int k) {
+  if (k <= 0) return;
+  if (!fusable(s)) {
+    for (int j = 0; j < k; j++) enqueue_step(s);
+    return;
+  }
+  launch_update(s, 1);
+  launch_cells(s, 1);
+  for (int j = 1; j < k; j++) {
+    launch_forces_move(s);
+    launch_cells(s, 1);
+  }
   launch_forces(s);
 }
 
@@ -3002,7 +3150,7 @@ static int build_graph(bd_sys* s, int steps) {
   }
   hipGraph_t g = nullptr;
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < steps; k++) enqueue_step(s);
+  enqueue_steps(s, steps);
   HIP_TRY(hipStreamEndCapture(s->stream, &g));
   hipError_t e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -3050,7 +3198,7 @@ int bd_step(bd_sys* s, int32_t nsteps) {
       left -= G;
     }
   }
-  for (; left > 0; left--) enqueue_step(s);
+  enqueue_steps(s, left);
   HIP_TRY(hipGetLastError());
   s->ni += nsteps;
   return check_flags(s);
@@ -3293,6 +3441,25 @@ int bd_get_pass_split(const bd_sys* s, int32_t* split) {
   return BD_OK;
 }
 
+int bd_set_fused_update(bd_sys* s, int32_t on) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (on < 0 || on > 2) return fail(BD_E_ARG, "fused update must be 0, 1 or 2");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->graph) {
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_steps = 0;
+  }
+  s->fuse = on;
+  return BD_OK;
+}
+
+int bd_get_fused_update(const bd_sys* s, int32_t* on) {
+  if (!s || !on) return fail(BD_E_ARG, "null argument");
+  *on = fusable(s) ? 1 : 0;
+  return BD_OK;
+}
+
 int bd_set_defer_depth(bd_sys* s, int32_t depth) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (depth != 0 && depth != 16 && depth != 32) return fail(BD_E_ARG, "depth must be 0, 16 or 32");
@@ -3425,6 +3592,45 @@ int bd_kernel_times(bd_sys* s, int32_t nsteps, double* ms) {
   }
   for (int k = 0; k < 4; k++) (void)hipEventDestroy(ev[k]);
   if (e != hipSuccess) return fail(BD_E_HIP, std::string("bd_kernel_times: ") + hipGetErrorString(e));
+  return check_flags(s);
+}
+
+int bd_kernel_times_fused(bd_sys* s, int32_t nsteps, double* ms) {
+  if (!s || !ms || nsteps < 2) return fail(BD_E_ARG, "bad arguments (nsteps >= 2)");
+  if (!fusable(s)) return fail(BD_E_STATE, "the fused update does not apply to these settings");
+  int rc = require_positions(s);
+  if (rc != BD_OK) return rc;
+  if (!s->located) {
+    launch_bin(s);
+    launch_cells(s, 0);
+    launch_forces(s);
+    s->located = true;
+  }
+  hipEvent_t ev[3];
+  for (int k = 0; k < 3; k++) HIP_TRY(hipEventCreate(&ev[k]));
+  hipError_t e = hipSuccess;
+  launch_update(s, 1);
+  for (int i = 0; i < nsteps && e == hipSuccess; i++) {
+    const bool last = i + 1 == nsteps;
+    e = hipEventRecord(ev[0], s->stream);
+    launch_cells(s, 1);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
+    if (last)
+      launch_forces(s);
+    else
+      launch_forces_move(s);
+    if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
+    ms[0] += (double)t;
+    if (e == hipSuccess && !last) e = hipEventElapsedTime(&t, ev[1], ev[2]);
+    if (!last) ms[1] += (double)t;
+    s->ni++;
+  }
+  for (int k = 0; k < 3; k++) (void)hipEventDestroy(ev[k]);
+  if (e != hipSuccess)
+    return fail(BD_E_HIP, std::string("bd_kernel_times_fused: ") + hipGetErrorString(e));
   return check_flags(s);
 }
 

@@ -225,6 +225,18 @@ class Engine:
         check(self.lib.bd_get_pass_split(self.h, C.byref(v)))
         return v.value
 
+    def set_fused_update(self, mode):
+        """Update of each step after the first inside the force kernel of the
+        step before: 0 never, 1 wherever the shape allows, 2 (default) with a
+        pass split only."""
+        check(self.lib.bd_set_fused_update(self.h, int(mode)))
+
+    def fused_update(self):
+        """Whether the fused update applies with the current settings."""
+        v = C.c_int32()
+        check(self.lib.bd_get_fused_update(self.h, C.byref(v)))
+        return v.value
+
     def set_defer_depth(self, depth):
         """Deferred pair terms of the row walk: 0, 16 or 32 LDS entries."""
         check(self.lib.bd_set_defer_depth(self.h, int(depth)))
@@ -255,6 +267,13 @@ class Engine:
         """ms spent in (update, cells, forces) over nsteps eager steps."""
         ms = np.zeros(3)
         check(self.lib.bd_kernel_times(self.h, int(nsteps), _dptr(ms)))
+        return ms
+
+    def kernel_times_fused(self, nsteps):
+        """ms spent in (cells, forces that also move the particles) over
+        nsteps eager steps of the fused path (nsteps - 1 fused launches)."""
+        ms = np.zeros(2)
+        check(self.lib.bd_kernel_times_fused(self.h, int(nsteps), _dptr(ms)))
         return ms
 
     def rng_lane(self, lane, r=0):

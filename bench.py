@@ -38,6 +38,7 @@ FLOP_PAIR = 22             # FP64 operations per ordered pair in the g(r) scan
 # WRITE_SIZE runs in profiles/k_forces_traffic_pmc.txt, scaled by a copy kernel of known size
 TRAFFIC_FORCES_PLAIN = 6.35e6
 TRAFFIC_FORCES_BATCH = 1.490e8
+TRAFFIC_FUSED_PLAIN = None  # the fused force kernel of the plain system: not yet counted
 
 
 def log(msg):
@@ -63,7 +64,7 @@ def pick_shape(eng, shape, trials):
         eng.set_pass_split(shape[3] if len(shape) > 3 else 0)
         brick = shape[0]
     return {"brick": brick, "walk": ["rows", "slabs", "chunks2", "chunks4"][eng.walk()],
-            "pass_split": eng.pass_split(),
+            "pass_split": eng.pass_split(), "fused_update": bool(eng.fused_update()),
             "defer_depth": eng.defer_depth(),
             "chosen_by": "autotuner" if shape is None else "--shape"}
 
@@ -89,6 +90,9 @@ def run_plain(steps, warmup, graph_steps=16, shape=None):
     dt = time.perf_counter() - t0
     ksteps = min(200, steps)
     ms = eng.kernel_times(ksteps) / ksteps
+    fused_ms = None
+    if eng.fused_update():
+        fused_ms = (eng.kernel_times_fused(ksteps + 1) / [ksteps + 1, ksteps]).tolist()
     obs = eng.observables(7.0)
     eng.gder_counts(128, 5.0)  # warm-up
     t0 = time.perf_counter()
@@ -99,7 +103,7 @@ def run_plain(steps, warmup, graph_steps=16, shape=None):
     eng.close()
     return {"N": nb + ns, "side": side, "seconds": dt, "rate": (nb + ns) * steps / dt,
             "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p, "shape": shape,
-            "gder_s": gder_s}
+            "gder_s": gder_s, "fused_ms": fused_ms}
 
 
 def run_batch(steps, warmup, graph_steps=16, shape=None):
@@ -262,8 +266,15 @@ def main():
         pool.join()
         log(f"[bench] cpu batch baseline: {cpu_batch['value']:.4e} particle-steps/s")
     N = plain["N"]
-    f_ms = plain["kernel_ms"][2]
-    achieved = B_FORCES * N / (f_ms * 1e-3)
+    # the dominant kernel as the timed steps run it: the force kernel, which
+    # with the fused update also moves the particles (forces + update bytes)
+    kname = {0: "k_forces", 1: "k_forces_pair", 2: "k_forces_quad"}[plain["shape"]["pass_split"]]
+    if plain["fused_ms"] is not None:
+        f_ms, f_bytes, traffic = plain["fused_ms"][1], B_FORCES + B_UPDATE, TRAFFIC_FUSED_PLAIN
+        kname += " (forces + fused update)"
+    else:
+        f_ms, f_bytes, traffic = plain["kernel_ms"][2], B_FORCES, TRAFFIC_FORCES_PLAIN
+    achieved = f_bytes * N / (f_ms * 1e-3)
     out = {
         "metric": "particle-steps/s at N=32768 phi=0.58 (plain single system)",
         "value": plain["rate"],
@@ -281,13 +292,14 @@ def main():
                                "T 1.0 -> 0.1", "N": N, "side": round(plain["side"], 4),
                    "potential": "wca", "cells": "23^3 big + 32^3 small, ntags 8",
                    "force_kernel": plain["shape"]},
-        "roofline": {"bound": "hbm", "kernel": "k_forces", "achieved": achieved / 1e9,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": TRAFFIC_FORCES_PLAIN, "traffic_unit": "B per launch (rocprofv3)",
-                     "bytes_per_particle": B_FORCES,
+                     "traffic": traffic, "traffic_unit": "B per launch (rocprofv3)",
+                     "bytes_per_particle": f_bytes,
                      "avg_launch_ms": f_ms},
         "kernel_ms": {"update": plain["kernel_ms"][0], "cells": plain["kernel_ms"][1],
-                      "forces": plain["kernel_ms"][2]},
+                      "forces": plain["kernel_ms"][2],
+                      "forces_with_update": plain["fused_ms"][1] if plain["fused_ms"] else None},
         "step_hbm_frac": B_STEP * plain["rate"] / HBM_PEAK,
         "gder": {"pairs_per_s": N * (N - 1) / plain["gder_s"],
                  "GFLOP_s": FLOP_PAIR * N * (N - 1) / plain["gder_s"] / 1e9,

@@ -193,6 +193,18 @@ int bd_get_walk(const bd_sys* s, int32_t* walk);
 int bd_set_pass_split(bd_sys* s, int32_t split);
 int bd_get_pass_split(const bd_sys* s, int32_t* split);
 
+/* Fused update (no reference counterpart): bd_step runs the update of each
+ * step after the first (update_ermak.cu:91-160) inside the force kernel of
+ * the step before, one launch fewer per step; same operations on the same
+ * values, bit-identical.  It can apply to the thread-per-particle slab walks
+ * (BD_WALK_SLABS, _CHUNK2, _CHUNK4) in reference order when every replica
+ * has one particle per rannew64 lane (rng_lanes 0 or >= N).  mode 0: never;
+ * 1: wherever it can; 2 (default): with a pass split at a nonzero deferral
+ * depth only (where it is faster).  bd_get_fused_update reports whether it
+ * applies with the current settings (0 or 1). */
+int bd_set_fused_update(bd_sys* s, int32_t mode);
+int bd_get_fused_update(const bd_sys* s, int32_t* on);
+
 /* Autotuner (no reference counterpart): measures the force kernel for every
  * shape (row, slab and chunked slab walks at deferral depths 0, 16 and 32,
  * the slab walks also with the pass split; bricks 1..4; all bit-identical by
@@ -232,6 +244,12 @@ int bd_checkpoint_load(bd_sys* s, const char* path);
  * each; adds the elapsed milliseconds of the update, cells and forces kernels
  * into ms[0], ms[1], ms[2]. */
 int bd_kernel_times(bd_sys* s, int32_t nsteps, double* ms);
+/* Timing of the fused path (no reference counterpart): nsteps >= 2 steps as
+ * bd_step runs them with the fused update, kernels launched one by one with
+ * HIP events around each; adds the milliseconds of the nsteps cells kernels
+ * into ms[0] and of the nsteps - 1 force kernels that also move the particles
+ * into ms[1].  BD_E_STATE when the fused update does not apply. */
+int bd_kernel_times_fused(bd_sys* s, int32_t nsteps, double* ms);
 
 /* ---- rannew64 lanes ---- */
 /* State of one lane of a replica: ring[250] (slot order) and the LCG word;

@@ -202,6 +202,40 @@ def test_pass_split_column_overflow(config1_positions):
     o.close()
 
 
+@pytest.mark.parametrize("split", [0, 1, 2])
+def test_fused_update_bit_identical(config1_positions, split):
+    """The update run inside the force kernel (default) against separate
+    update launches: positions, raw positions, forces, rannew64 lanes and cell
+    lists all equal after graph replays and eager steps, through a quench."""
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, T0=1.0, Tf=0.3, t0=0.0, tf=0.004, seed=11)
+    runs = []
+    for fused in (0, 1):
+        e = Engine(p)
+        e.set_walk(abi.BD_WALK_CHUNK4)
+        e.set_defer_depth(16)
+        e.set_pass_split(split)
+        e.set_fused_update(fused)
+        assert e.fused_update() == fused
+        e.set_positions(x)
+        e.step(37)  # two 16-step graphs and 5 eager steps
+        runs.append([e.positions(), e.raw_positions(), *e.forces(), *e.rng_lane(5),
+                     *e.rng_lane(1999), *e.cells(0), *e.cells(1)])
+        e.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    e = Engine(p)
+    e.set_walk(abi.BD_WALK_CHUNK4)
+    e.set_defer_depth(16)
+    e.set_pass_split(split)
+    assert e.fused_update() == (1 if split else 0)  # default mode 2: with a pass split
+    e.close()
+    e = Engine(make_params(1000, 1000, side=side, seed=11, rng_lanes=512))
+    e.set_fused_update(1)
+    assert e.fused_update() == 0  # lanes serve two particles each: separate update
+    e.close()
+
+
 @pytest.fixture(scope="module")
 def plain_positions():
     """N=32768 (16384 + 16384) at phi=0.58, no overlaps: the bench workload."""
