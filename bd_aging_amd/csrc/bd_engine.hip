@@ -38,6 +38,7 @@
 //                 cross-species stencil in the reference's traversal order
 //                 (get_forces.cu:297-558), written (not accumulated)
 #include <hip/hip_cooperative_groups.h>
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,6 +46,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/bd_abi.h"
@@ -2200,6 +2202,9 @@ struct bd_sys {
   // update inside the force kernel: 0 never, 1 wherever the shape allows it,
   // 2 (auto) with the multi-wavefront pass splits only
   int fuse = 2;
+  // kernel timing: when set, launch() has the runtime stamp these at the
+  // kernel's start and end
+  hipEvent_t tev[2] = {nullptr, nullptr};
   bool defer_ok = true;  // every grid's slot index fits the 26-bit record
   std::vector<int4> tbl_split_h;
   int4* tbl_split = nullptr;
@@ -2209,16 +2214,37 @@ struct bd_sys {
 
 namespace {
 
+// Every kernel of a step is launched through launch().  With timing events
+// set on the engine (bd_kernel_times, bd_kernel_times_fused) the runtime
+// stamps them at the kernel's own start and end (hipExtLaunchKernel), so the
+// figures are kernel durations, without the dispatch gaps that an event pair
+// recorded around the launch would add.
+template <typename... P, typename... A>
+void launch(bd_sys* s, void (*k)(P...), int nblk, int nthr, A... a) {
+  std::tuple<P...> args{a...};
+  std::apply(
+      [&](auto&... e) {
+        void* ptrs[] = {static_cast<void*>(&e)...};
+        if (s->tev[0])
+          (void)hipExtLaunchKernel(reinterpret_cast<const void*>(k), dim3(nblk), dim3(nthr), ptrs,
+                                   0, s->stream, s->tev[0], s->tev[1], 0);
+        else
+          (void)hipLaunchKernel(reinterpret_cast<const void*>(k), dim3(nblk), dim3(nthr), ptrs, 0,
+                                s->stream);
+      },
+      args);
+}
+
 int launch_bin(bd_sys* s) {
   const int nb = (int)s->tbl_part_h.size();
-  k_bin<<<nb, kBlk, 0, s->stream>>>(s->reps_d, s->tbl_part, s->px, s->py, s->pz, s->nocup,
+  launch(s, k_bin, nb, kBlk, s->reps_d, s->tbl_part, s->px, s->py, s->pz, s->nocup,
                                     s->ins, s->flags);
   return BD_OK;
 }
 
 void launch_update(bd_sys* s, int do_bin) {
   const int nb = (int)s->tbl_lane_h.size();
-  k_update_bin<<<nb, kBlk, 0, s->stream>>>(s->reps_d, s->tbl_lane, s->px, s->py, s->pz, s->rx,
+  launch(s, k_update_bin, nb, kBlk, s->reps_d, s->tbl_lane, s->px, s->py, s->pz, s->rx,
                                            s->ry, s->rz, s->fx, s->fy, s->fz, s->ring, s->lcg,
                                            s->draws, s->step, s->lanes_total, s->nocup, s->ins,
                                            s->flags, do_bin);
@@ -2226,7 +2252,7 @@ void launch_update(bd_sys* s, int do_bin) {
 
 void launch_cells(bd_sys* s, int advance) {
   const int nb = (int)s->tbl_cell_h.size();
-  k_cells<<<nb, kBlk, 0, s->stream>>>(s->reps_d, s->tbl_cell, s->px, s->py, s->pz, s->nocup,
+  launch(s, k_cells, nb, kBlk, s->reps_d, s->tbl_cell, s->px, s->py, s->pz, s->nocup,
                                       s->cnt, s->ins, s->rs, s->cpos, s->flags, s->step,
                                       s->draws, advance);
 }
@@ -2242,17 +2268,17 @@ void launch_forces_k(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* 
   const MoveArgs mv = move_args(s);
   switch (s->pot_kind) {
     case BD_POT_WCA:
-      k_forces<BD_POT_WCA, NEIGH, DEPTH, WALK, UPD><<<nblk, kBlk, 0, s->stream>>>(
+      launch(s, k_forces<BD_POT_WCA, NEIGH, DEPTH, WALK, UPD>, nblk, kBlk,
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, ncounts, nlist, maxn, mv);
       break;
     case BD_POT_WCA_MODIFIED:
-      k_forces<BD_POT_WCA_MODIFIED, NEIGH, DEPTH, WALK, UPD><<<nblk, kBlk, 0, s->stream>>>(
+      launch(s, k_forces<BD_POT_WCA_MODIFIED, NEIGH, DEPTH, WALK, UPD>, nblk, kBlk,
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, ncounts, nlist, maxn, mv);
       break;
     default:
-      k_forces<BD_POT_HARMONIC, NEIGH, DEPTH, WALK, UPD><<<nblk, kBlk, 0, s->stream>>>(
+      launch(s, k_forces<BD_POT_HARMONIC, NEIGH, DEPTH, WALK, UPD>, nblk, kBlk,
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, ncounts, nlist, maxn, mv);
       break;
@@ -2274,17 +2300,17 @@ void launch_pair_k(bd_sys* s, const int4* tbl, int nblk) {
   const MoveArgs mv = move_args(s);
   switch (s->pot_kind) {
     case BD_POT_WCA:
-      k_forces_pair<BD_POT_WCA, DEPTH, WALK, UPD><<<nblk, 2 * kBlk, 0, s->stream>>>(
+      launch(s, k_forces_pair<BD_POT_WCA, DEPTH, WALK, UPD>, nblk, 2 * kBlk,
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, mv);
       break;
     case BD_POT_WCA_MODIFIED:
-      k_forces_pair<BD_POT_WCA_MODIFIED, DEPTH, WALK, UPD><<<nblk, 2 * kBlk, 0, s->stream>>>(
+      launch(s, k_forces_pair<BD_POT_WCA_MODIFIED, DEPTH, WALK, UPD>, nblk, 2 * kBlk,
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, mv);
       break;
     default:
-      k_forces_pair<BD_POT_HARMONIC, DEPTH, WALK, UPD><<<nblk, 2 * kBlk, 0, s->stream>>>(
+      launch(s, k_forces_pair<BD_POT_HARMONIC, DEPTH, WALK, UPD>, nblk, 2 * kBlk,
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, mv);
       break;
@@ -2296,17 +2322,17 @@ void launch_quad_k(bd_sys* s, const int4* tbl, int nblk) {
   const MoveArgs mv = move_args(s);
   switch (s->pot_kind) {
     case BD_POT_WCA:
-      k_forces_quad<BD_POT_WCA, DEPTH, WALK, UPD><<<nblk, 4 * kBlk, 0, s->stream>>>(
+      launch(s, k_forces_quad<BD_POT_WCA, DEPTH, WALK, UPD>, nblk, 4 * kBlk,
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, mv);
       break;
     case BD_POT_WCA_MODIFIED:
-      k_forces_quad<BD_POT_WCA_MODIFIED, DEPTH, WALK, UPD><<<nblk, 4 * kBlk, 0, s->stream>>>(
+      launch(s, k_forces_quad<BD_POT_WCA_MODIFIED, DEPTH, WALK, UPD>, nblk, 4 * kBlk,
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, mv);
       break;
     default:
-      k_forces_quad<BD_POT_HARMONIC, DEPTH, WALK, UPD><<<nblk, 4 * kBlk, 0, s->stream>>>(
+      launch(s, k_forces_quad<BD_POT_HARMONIC, DEPTH, WALK, UPD>, nblk, 4 * kBlk,
           s->reps_d, tbl, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir,
           s->pot, mv);
       break;
@@ -2360,17 +2386,17 @@ void launch_brick_t(bd_sys* s) {
   const int nblk = (int)s->tbl_brick_h.size();
   switch (s->pot_kind) {
     case BD_POT_WCA:
-      k_forces_brick<BD_POT_WCA, CAP><<<nblk, kBrickThreads, 0, s->stream>>>(
+      launch(s, k_forces_brick<BD_POT_WCA, CAP>, nblk, kBrickThreads,
           s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy,
           s->fz, s->vir, s->pot, s->flags);
       break;
     case BD_POT_WCA_MODIFIED:
-      k_forces_brick<BD_POT_WCA_MODIFIED, CAP><<<nblk, kBrickThreads, 0, s->stream>>>(
+      launch(s, k_forces_brick<BD_POT_WCA_MODIFIED, CAP>, nblk, kBrickThreads,
           s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy,
           s->fz, s->vir, s->pot, s->flags);
       break;
     default:
-      k_forces_brick<BD_POT_HARMONIC, CAP><<<nblk, kBrickThreads, 0, s->stream>>>(
+      launch(s, k_forces_brick<BD_POT_HARMONIC, CAP>, nblk, kBrickThreads,
           s->reps_d, s->tbl_brick, s->brick, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy,
           s->fz, s->vir, s->pot, s->flags);
       break;
@@ -2382,17 +2408,17 @@ void launch_split_t(bd_sys* s) {
   const int nblk = (int)s->tbl_split_h.size();
   switch (s->pot_kind) {
     case BD_POT_WCA:
-      k_forces_split<BD_POT_WCA, G><<<nblk, kBlk, 0, s->stream>>>(
+      launch(s, k_forces_split<BD_POT_WCA, G>, nblk, kBlk,
           s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz,
           s->vir, s->pot);
       break;
     case BD_POT_WCA_MODIFIED:
-      k_forces_split<BD_POT_WCA_MODIFIED, G><<<nblk, kBlk, 0, s->stream>>>(
+      launch(s, k_forces_split<BD_POT_WCA_MODIFIED, G>, nblk, kBlk,
           s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz,
           s->vir, s->pot);
       break;
     default:
-      k_forces_split<BD_POT_HARMONIC, G><<<nblk, kBlk, 0, s->stream>>>(
+      launch(s, k_forces_split<BD_POT_HARMONIC, G>, nblk, kBlk,
           s->reps_d, s->tbl_split, s->px, s->py, s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz,
           s->vir, s->pot);
       break;
@@ -3561,6 +3587,34 @@ int bd_checkpoint_load(bd_sys* s, const char* path) {
   return BD_OK;
 }
 
+// Kernel durations for the timing entry points: each launch between
+// timed_begin(k) and timed_end() is stamped into event pair k.
+struct KernelTimer {
+  bd_sys* s;
+  std::vector<hipEvent_t> ev;
+  hipError_t e = hipSuccess;
+  KernelTimer(bd_sys* sys, int pairs) : s(sys), ev(2 * pairs, nullptr) {
+    for (auto& x : ev)
+      if (e == hipSuccess) e = hipEventCreate(&x);
+  }
+  ~KernelTimer() {
+    s->tev[0] = s->tev[1] = nullptr;
+    for (auto& x : ev)
+      if (x) (void)hipEventDestroy(x);
+  }
+  void begin(int k) {
+    s->tev[0] = ev[2 * k];
+    s->tev[1] = ev[2 * k + 1];
+  }
+  void end() { s->tev[0] = s->tev[1] = nullptr; }
+  // adds pair k's milliseconds to *acc (after a synchronisation)
+  void add(int k, double* acc) {
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]);
+    *acc += (double)t;
+  }
+};
+
 int bd_kernel_times(bd_sys* s, int32_t nsteps, double* ms) {
   if (!s || !ms || nsteps < 0) return fail(BD_E_ARG, "bad arguments");
   int rc = require_positions(s);
@@ -3571,27 +3625,21 @@ int bd_kernel_times(bd_sys* s, int32_t nsteps, double* ms) {
     launch_forces(s);
     s->located = true;
   }
-  hipEvent_t ev[4];
-  for (int k = 0; k < 4; k++) HIP_TRY(hipEventCreate(&ev[k]));
-  hipError_t e = hipSuccess;
-  for (int i = 0; i < nsteps && e == hipSuccess; i++) {
-    e = hipEventRecord(ev[0], s->stream);
+  KernelTimer kt(s, 3);
+  for (int i = 0; i < nsteps && kt.e == hipSuccess; i++) {
+    kt.begin(0);
     launch_update(s, 1);
-    if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
+    kt.begin(1);
     launch_cells(s, 1);
-    if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
+    kt.begin(2);
     launch_forces(s);
-    if (e == hipSuccess) e = hipEventRecord(ev[3], s->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
-    for (int k = 0; k < 3 && e == hipSuccess; k++) {
-      float t = 0.f;
-      e = hipEventElapsedTime(&t, ev[k], ev[k + 1]);
-      ms[k] += (double)t;
-    }
+    kt.end();
+    if (kt.e == hipSuccess) kt.e = hipStreamSynchronize(s->stream);
+    for (int k = 0; k < 3; k++) kt.add(k, &ms[k]);
     s->ni++;
   }
-  for (int k = 0; k < 4; k++) (void)hipEventDestroy(ev[k]);
-  if (e != hipSuccess) return fail(BD_E_HIP, std::string("bd_kernel_times: ") + hipGetErrorString(e));
+  if (kt.e != hipSuccess)
+    return fail(BD_E_HIP, std::string("bd_kernel_times: ") + hipGetErrorString(kt.e));
   return check_flags(s);
 }
 
@@ -3606,31 +3654,25 @@ int bd_kernel_times_fused(bd_sys* s, int32_t nsteps, double* ms) {
     launch_forces(s);
     s->located = true;
   }
-  hipEvent_t ev[3];
-  for (int k = 0; k < 3; k++) HIP_TRY(hipEventCreate(&ev[k]));
-  hipError_t e = hipSuccess;
+  KernelTimer kt(s, 2);
   launch_update(s, 1);
-  for (int i = 0; i < nsteps && e == hipSuccess; i++) {
+  for (int i = 0; i < nsteps && kt.e == hipSuccess; i++) {
     const bool last = i + 1 == nsteps;
-    e = hipEventRecord(ev[0], s->stream);
+    kt.begin(0);
     launch_cells(s, 1);
-    if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
+    kt.begin(1);
     if (last)
       launch_forces(s);
     else
       launch_forces_move(s);
-    if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
-    float t = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
-    ms[0] += (double)t;
-    if (e == hipSuccess && !last) e = hipEventElapsedTime(&t, ev[1], ev[2]);
-    if (!last) ms[1] += (double)t;
+    kt.end();
+    if (kt.e == hipSuccess) kt.e = hipStreamSynchronize(s->stream);
+    kt.add(0, &ms[0]);
+    if (!last) kt.add(1, &ms[1]);
     s->ni++;
   }
-  for (int k = 0; k < 3; k++) (void)hipEventDestroy(ev[k]);
-  if (e != hipSuccess)
-    return fail(BD_E_HIP, std::string("bd_kernel_times_fused: ") + hipGetErrorString(e));
+  if (kt.e != hipSuccess)
+    return fail(BD_E_HIP, std::string("bd_kernel_times_fused: ") + hipGetErrorString(kt.e));
   return check_flags(s);
 }
 
@@ -3834,3 +3876,4 @@ int bd_observables(bd_sys* s, int32_t replica, double q, bd_obs* out) {
 }
 
 }  // extern "C"
+

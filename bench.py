@@ -34,11 +34,13 @@ B_BIN = 36                 # algorithmic bytes per particle, binning pass
 B_UPDATE = 120             # algorithmic bytes per particle, update pass (ring excluded)
 B_STEP = 228               # algorithmic bytes per particle-step (bin 36 + forces 72 + update 120)
 FLOP_PAIR = 22             # FP64 operations per ordered pair in the g(r) scan
-# HBM bytes per force launch (slab walk, deferral depth 16) from the rocprofv3 FETCH_SIZE /
-# WRITE_SIZE runs in profiles/k_forces_traffic_pmc.txt, scaled by a copy kernel of known size
-TRAFFIC_FORCES_PLAIN = 6.35e6
-TRAFFIC_FORCES_BATCH = 1.490e8
-TRAFFIC_FUSED_PLAIN = None  # the fused force kernel of the plain system: not yet counted
+# HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs in
+# profiles/k_forces_traffic_pmc.txt, scaled by a copy kernel of known size:
+# plain N=32768 with the four-wavefront chunked walk (forces only, and forces
+# with the fused update), batch with the slab walk (both at deferral depth 16)
+TRAFFIC_FORCES_PLAIN = 6.60e6
+TRAFFIC_FUSED_PLAIN = 1.465e7
+TRAFFIC_FORCES_BATCH = 1.489e8
 
 
 def log(msg):
@@ -229,7 +231,8 @@ def main():
     ap.add_argument("--stress-steps", type=int, default=1000,
                     help="steps of the N=262144 stress configuration (0 skips it)")
     ap.add_argument("--shape", default=None,
-                    help="force kernel as BRICK,DEPTH[,WALK] (e.g. 0,16,1) instead of autotuning")
+                    help="force kernel as BRICK,DEPTH[,WALK[,SPLIT]] (e.g. 0,16,3,2) instead of "
+                         "autotuning")
     ap.add_argument("--graph-steps", type=int, default=16,
                     help="steps per captured HIP graph; 0 launches every kernel eagerly "
                          "(same kernels, for kernel tracing)")
