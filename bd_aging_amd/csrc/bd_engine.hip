@@ -207,17 +207,33 @@ __global__ void __launch_bounds__(kBlk)
 }
 
 // Four rannew64 draws of one lane (rannew64.cu:99-126): ring slot np and
-// its lag partner, the LCG; np steps down the ring.
+// its lag partner, the LCG; np steps down the ring.  The four slots np..np-3
+// and their partners (np+103..np+100, mod 250) never coincide, so all eight
+// ring words are loaded before any is rewritten: the same values as the
+// serial recurrence, with one memory latency instead of four.
+static_assert(bd::kLagGap > 3 && bd::kRingLen - bd::kLagGap > 3,
+              "a draw block's slots and lag partners must not overlap");
 __device__ inline void draw4(uint64_t* rl, size_t stride, int& np, uint64_t& lcg,
                              uint64_t d[4]) {
+  int sp[4], sq[4];
+  uint64_t a[4], b[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int nq = np >= bd::kLagGap ? np - bd::kLagGap : np + (bd::kRingLen - bd::kLagGap);
-    const uint64_t x = rl[(size_t)np * stride] - rl[(size_t)nq * stride];
-    rl[(size_t)np * stride] = x;
+    sp[j] = np;
+    sq[j] = np >= bd::kLagGap ? np - bd::kLagGap : np + (bd::kRingLen - bd::kLagGap);
+    np = np == 0 ? bd::kRingLen - 1 : np - 1;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    a[j] = rl[(size_t)sp[j] * stride];
+    b[j] = rl[(size_t)sq[j] * stride];
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const uint64_t x = a[j] - b[j];
+    rl[(size_t)sp[j] * stride] = x;
     lcg = bd::kLcgA * lcg + bd::kLcgC;
     d[j] = x - lcg;
-    np = np == 0 ? bd::kRingLen - 1 : np - 1;
   }
 }
 
@@ -1355,14 +1371,14 @@ __device__ inline bool walk_pass(const RepDev& R, int i, int pass, double x, dou
   return true;
 }
 
-// The five terms one recorded pair adds (pair_add's operations, split
-// from the accumulation): force components, virial term, energy term.
+// The five terms one recorded pair adds (pair_add's operations, split from
+// the accumulation): force components, virial term, energy term; qc is the
+// partner's slot, already loaded.
 template <int POT>
-__device__ inline void pair_terms(const RepDev& R, double x, double y, double z, const Defer& d,
-                                  unsigned int wc, double& qx, double& qy, double& qz,
-                                  double& wv, double& e) {
+__device__ inline void pair_terms_q(const RepDev& R, double x, double y, double z, const Defer& d,
+                                    unsigned int wc, const double4& qc, double& qx, double& qy,
+                                    double& qz, double& wv, double& e) {
   const double L = R.side;
-  const double4 qc = ((wc & kPassBit) ? d.cp1 : d.cp0)[wc & kSlotMask];
   const unsigned int cx = (wc >> kSlotBits) & 3u, cy = (wc >> (kSlotBits + 2)) & 3u,
                      cz = (wc >> (kSlotBits + 4)) & 3u;
   const double dx = (qc.x - x) + (cx == 0u ? 0.0 : (cx == 1u ? -L : L));
@@ -1378,21 +1394,42 @@ __device__ inline void pair_terms(const RepDev& R, double x, double y, double z,
   wv = 0.5 * (dx * qx + dy * qy + dz * qz);
 }
 
-// Pair terms of the records e0 .. e1-1 of d's column, added in order.
-template <int POT>
+__device__ inline double4 record_slot(const Defer& d, unsigned int wc) {
+  return ((wc & kPassBit) ? d.cp1 : d.cp0)[wc & kSlotMask];
+}
+
+// Pair terms of the records e0 .. e1-1 of d's column, added in order; the
+// partner slots are loaded U at a time before they are used (one memory
+// latency per U records).
+template <int POT, int U = 1>
 __device__ inline void drain_range(const RepDev& R, double x, double y, double z, const Defer& d,
                                    int e0, int e1, double& fx, double& fy, double& fz,
                                    double& vir, double& pot) {
-  for (int e = e0; e < e1; e++) {
-    double qx, qy, qz, wv, en;
-    pair_terms<POT>(R, x, y, z, d, d.buf[e * kBlk], qx, qy, qz, wv, en);
-    pot = pot + en;
-    fx += qx;
-    fy += qy;
-    fz += qz;
-    vir -= wv;
+  for (int e = e0; e < e1; e += U) {
+    unsigned int wc[U];
+    double4 qc[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) wc[k] = d.buf[(e + k < e1 ? e + k : e) * kBlk];
+#pragma unroll
+    for (int k = 0; k < U; k++) qc[k] = record_slot(d, wc[k]);
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      if (e + k < e1) {
+        double qx, qy, qz, wv, en;
+        pair_terms_q<POT>(R, x, y, z, d, wc[k], qc[k], qx, qy, qz, wv, en);
+        pot = pot + en;
+        fx += qx;
+        fy += qy;
+        fz += qz;
+        vir -= wv;
+      }
+    }
   }
 }
+
+// Records whose partner slots the split kernels load together before
+// evaluating them (their occupancy is set by LDS, so the registers are free).
+constexpr int kDrainBatch = 4;
 
 // Stencil passes split over two wavefronts (no reference counterpart): for
 // the same 64 particles, wavefront 0 walks the same-species stencil and
@@ -1463,10 +1500,11 @@ __global__ void __launch_bounds__(2 * kBlk)
     // grid ends its walk there, before the final drain
     const int n1 = n1s[lane];
     if (n1 != -2) {
-      drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
+      drain_range<POT, kDrainBatch>(R, x, y, z, df, 0, df.nd, fx, fy, fz, v, u);
+      df.nd = 0;
       if (n1 >= 0) {
         Defer d1{df.cp0, df.cp1, df.sigma0, df.sigma1, dbuf + DEPTH * kBlk + lane, n1};
-        drain<POT>(R, x, y, z, d1, fx, fy, fz, v, u);
+        drain_range<POT, kDrainBatch>(R, x, y, z, d1, 0, n1, fx, fy, fz, v, u);
       } else if (walk_pass<POT, DEPTH, WALK, false>(R, i, 1, x, y, z, rs, cpos, df, seg, fx, fy,
                                                     fz, v, u)) {
         drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
@@ -1536,14 +1574,26 @@ __global__ void __launch_bounds__(4 * kBlk)
         nw = df.nd <= DEPTH ? df.nd : -1;
         double* tq = terms + (w - 1) * kTermCap * 5 * kBlk + lane;
         const int m = nw < kTermCap ? nw : kTermCap;
-        for (int e = 0; e < m; e++) {
-          double qx, qy, qz, wv, en;
-          pair_terms<POT>(R, x, y, z, df, df.buf[e * kBlk], qx, qy, qz, wv, en);
-          tq[(5 * e + 0) * kBlk] = qx;
-          tq[(5 * e + 1) * kBlk] = qy;
-          tq[(5 * e + 2) * kBlk] = qz;
-          tq[(5 * e + 3) * kBlk] = wv;
-          tq[(5 * e + 4) * kBlk] = en;
+        // all partner slots loaded before the first is used
+        unsigned int wc[kTermCap];
+        double4 qc[kTermCap];
+#pragma unroll
+        for (int e = 0; e < kTermCap; e++)
+          if (e < m) wc[e] = df.buf[e * kBlk];
+#pragma unroll
+        for (int e = 0; e < kTermCap; e++)
+          if (e < m) qc[e] = record_slot(df, wc[e]);
+#pragma unroll
+        for (int e = 0; e < kTermCap; e++) {
+          if (e < m) {
+            double qx, qy, qz, wv, en;
+            pair_terms_q<POT>(R, x, y, z, df, wc[e], qc[e], qx, qy, qz, wv, en);
+            tq[(5 * e + 0) * kBlk] = qx;
+            tq[(5 * e + 1) * kBlk] = qy;
+            tq[(5 * e + 2) * kBlk] = qz;
+            tq[(5 * e + 3) * kBlk] = wv;
+            tq[(5 * e + 4) * kBlk] = en;
+          }
         }
       } else {
         nw = -2;
@@ -1558,7 +1608,10 @@ __global__ void __launch_bounds__(4 * kBlk)
     if (R.fast) {
       ok = walk_pass<POT, DEPTH, WALK, false>(R, i, 0, x, y, z, rs, cpos, df, seg, fx, fy, fz, v,
                                               u, 0);
-      if (ok) drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
+      if (ok) {
+        drain_range<POT, kDrainBatch>(R, x, y, z, df, 0, df.nd, fx, fy, fz, v, u);
+        df.nd = 0;
+      }
     } else {
       forces_global<POT, false>(R, i, x, y, z, rs, cpos, fx, fy, fz, v, u);
     }
@@ -1585,7 +1638,7 @@ __global__ void __launch_bounds__(4 * kBlk)
         if (nw > kTermCap) {
           const Defer dk{df.cp0, df.cp1, df.sigma0, df.sigma1, dbuf + k * DEPTH * kBlk + lane,
                          nw};
-          drain_range<POT>(R, x, y, z, dk, kTermCap, nw, fx, fy, fz, v, u);
+          drain_range<POT, kDrainBatch>(R, x, y, z, dk, kTermCap, nw, fx, fy, fz, v, u);
         }
       } else if (walk_pass<POT, DEPTH, WALK, false>(R, i, k >> 1, x, y, z, rs, cpos, df, seg, fx,
                                                     fy, fz, v, u, k & 1)) {
