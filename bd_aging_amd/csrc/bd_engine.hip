@@ -237,17 +237,39 @@ __device__ inline void draw4(uint64_t* rl, size_t stride, int& np, uint64_t& lcg
   }
 }
 
+// Neighbour-list bookkeeping of the moves (list walk only): raw positions
+// at the last list build, and per replica two rebuild flags used on
+// alternate steps (flag[2 r + parity]); a move that takes a particle more
+// than half the skin from its build position raises the flag of the step
+// whose forces come next.
+struct ListMove {
+  const double *rbx, *rby, *rbz;
+  int* lflag;
+  double half_skin2;
+};
+
+// Step parity of a replica's rebuild flags, from its draw counter (four
+// draws per lane slot and step; the counter advances in the binning, before
+// the forces): ahead = 0 for the forces of the current positions, 1 for the
+// move that produces the next ones.
+__device__ inline int list_parity(const RepDev& R, uint64_t draws, int ahead) {
+  return (int)(((draws / (4ULL * (uint64_t)R.per_lane)) + (uint64_t)ahead) & 1ULL);
+}
+
 // Ermak-McCammon move of replica-local particle i (update_ermak.cu:91-160)
 // from (x, y, z) with force (fx, fy, fz) and normals (xi, yi, zi) at
 // temperature T: raw and wrapped positions written, the wrapped one inserted
-// in its cell (do_bin).  Returns false for a non-finite result.
+// in its cell (do_bin).  With lm, the rebuild flag lflag[2 rep + par] is
+// raised for a move past half the skin.  Returns false for a non-finite
+// result.
 __device__ inline bool ermak_move(const RepDev& R, int rep, int i, double T, double x, double y,
                                   double z, double fx, double fy, double fz, double xi,
                                   double yi, double zi, double* __restrict__ px,
                                   double* __restrict__ py, double* __restrict__ pz,
                                   double* __restrict__ rx, double* __restrict__ ry,
                                   double* __restrict__ rz, int* __restrict__ nocup,
-                                  int* __restrict__ ins, int* __restrict__ flags, int do_bin) {
+                                  int* __restrict__ ins, int* __restrict__ flags, int do_bin,
+                                  const ListMove* lm = nullptr, int par = 0) {
   const int p = R.p_off + i;
   const double side = R.side, side_inv = R.side_inv, dt = R.dt;
   const double diameter = i < R.nb ? R.sigma[0] : R.sigma[1];
@@ -259,9 +281,14 @@ __device__ inline bool ermak_move(const RepDev& R, int rep, int i, double T, dou
   x = x + dx;
   y = y + dy;
   z = z + dz;
-  rx[p] = rx[p] + dx;
-  ry[p] = ry[p] + dy;
-  rz[p] = rz[p] + dz;
+  const double qx = rx[p] + dx, qy = ry[p] + dy, qz = rz[p] + dz;
+  rx[p] = qx;
+  ry[p] = qy;
+  rz[p] = qz;
+  if (lm) {
+    const double ex = qx - lm->rbx[p], ey = qy - lm->rby[p], ez = qz - lm->rbz[p];
+    if (!(ex * ex + ey * ey + ez * ez <= lm->half_skin2)) atomicOr(&lm->lflag[2 * rep + par], 1);
+  }
   x -= side * floor(side_inv * x);
   y -= side * floor(side_inv * y);
   z -= side * floor(side_inv * z);
@@ -286,7 +313,8 @@ __device__ __forceinline__ void k_update_bin_block(
                  const double* __restrict__ fz, uint64_t* __restrict__ ring,
                  uint64_t* __restrict__ lcgs, const uint64_t* __restrict__ draws,
                  const int64_t* __restrict__ step, int total_lanes, int* __restrict__ nocup,
-                 int* __restrict__ ins, int* __restrict__ flags, int do_bin) {
+                 int* __restrict__ ins, int* __restrict__ flags, int do_bin,
+                 ListMove lm = ListMove{nullptr, nullptr, nullptr, nullptr, 0.0}) {
   const int4 t = tbl[vb];
   if (t.x < 0) return;
   const int l = t.y + (int)threadIdx.x;
@@ -294,8 +322,10 @@ __device__ __forceinline__ void k_update_bin_block(
   const RepDev& R = reps[t.x];
   const int gl = R.lane_off + l;
   const double T = temperature_at(R, *step);
+  const uint64_t dr = draws[t.x];
+  const int par = list_parity(R, dr, 1);
   uint64_t lcg = lcgs[gl];
-  int np = bd::kRingLen - 1 - (int)(draws[t.x] % (uint64_t)bd::kRingLen);
+  int np = bd::kRingLen - 1 - (int)(dr % (uint64_t)bd::kRingLen);
   uint64_t* rl = ring + gl;
   const size_t stride = (size_t)total_lanes;
   bool bad = false;
@@ -308,7 +338,7 @@ __device__ __forceinline__ void k_update_bin_block(
     double xi, yi, zi;
     bd::gauss3(d[0], d[1], d[2], d[3], xi, yi, zi);
     if (!ermak_move(R, t.x, i, T, px[p], py[p], pz[p], fx[p], fy[p], fz[p], xi, yi, zi, px, py,
-                    pz, rx, ry, rz, nocup, ins, flags, do_bin))
+                    pz, rx, ry, rz, nocup, ins, flags, do_bin, lm.lflag ? &lm : nullptr, par))
       bad = true;
   }
   lcgs[gl] = lcg;
@@ -339,9 +369,9 @@ __global__ void __launch_bounds__(kBlk)
                  const double* __restrict__ fz, uint64_t* __restrict__ ring,
                  uint64_t* __restrict__ lcgs, const uint64_t* __restrict__ draws,
                  const int64_t* __restrict__ step, int total_lanes, int* __restrict__ nocup,
-                 int* __restrict__ ins, int* __restrict__ flags, int do_bin) {
+                 int* __restrict__ ins, int* __restrict__ flags, int do_bin, ListMove lm) {
   k_update_bin_block((int)blockIdx.x, reps, tbl, px, py, pz, rx, ry, rz, fx, fy, fz, ring, lcgs,
-                     draws, step, total_lanes, nocup, ins, flags, do_bin);
+                     draws, step, total_lanes, nocup, ins, flags, do_bin, lm);
 }
 
 // Row packing.  A wavefront packs rpw = 64 / n rows (x, y) of a grid with n
@@ -364,12 +394,17 @@ __device__ __forceinline__ void k_cells_block(
             const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
             const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
             int* __restrict__ flags, int64_t* __restrict__ step,
-            uint64_t* __restrict__ draws, int advance) {
+            uint64_t* __restrict__ draws, int advance, int* __restrict__ lflag = nullptr) {
   const int4 t = tbl[vb];
   if (advance && threadIdx.x == 0) {
     if (vb == 0) *step = *step + 1;
-    // each replica's draw counter by the workgroup of its first big-grid rows
-    if (t.x >= 0 && t.y == 0 && t.w == 0) draws[t.x] += 4ULL * (uint64_t)reps[t.x].per_lane;
+    // each replica's draw counter by the workgroup of its first big-grid rows,
+    // which also clears the list rebuild flag that the next move will set
+    if (t.x >= 0 && t.y == 0 && t.w == 0) {
+      const uint64_t d = draws[t.x];
+      if (lflag) lflag[2 * t.x + list_parity(reps[t.x], d, 0)] = 0;
+      draws[t.x] = d + 4ULL * (uint64_t)reps[t.x].per_lane;
+    }
   }
   if (t.x < 0) return;
   const int g = t.w;
@@ -455,9 +490,9 @@ __global__ void __launch_bounds__(kBlk)
             const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
             const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
             int* __restrict__ flags, int64_t* __restrict__ step,
-            uint64_t* __restrict__ draws, int advance) {
+            uint64_t* __restrict__ draws, int advance, int* __restrict__ lflag) {
   k_cells_block((int)blockIdx.x, reps, tbl, px, py, pz, nocup, cnt, ins, rs, cpos, flags, step,
-                draws, advance);
+                draws, advance, lflag);
 }
 
 // Distance along one axis from a point to the cell at offset d of its home
@@ -1249,6 +1284,7 @@ struct MoveArgs {
   int total_lanes;
   double *rx, *ry, *rz;
   int *nocup, *ins, *flags;
+  ListMove lm;  // lm.lflag == nullptr: no neighbour lists
 };
 
 // The fused move of particle i once its force is known (replicas with one
@@ -1259,7 +1295,8 @@ __device__ inline void fused_move(const RepDev& R, int rep, int i, double x, dou
                                   const MoveArgs& mv) {
   const double T = temperature_at(R, *mv.step);
   if (!ermak_move(R, rep, i, T, x, y, z, fx, fy, fz, xi, yi, zi, px, py, pz, mv.rx, mv.ry,
-                  mv.rz, mv.nocup, mv.ins, mv.flags, 1))
+                  mv.rz, mv.nocup, mv.ins, mv.flags, 1, mv.lm.lflag ? &mv.lm : nullptr,
+                  list_parity(R, mv.draws[rep], 1)))
     atomicOr(&mv.flags[rep], kFlagNonFinite);
 }
 
@@ -1654,6 +1691,197 @@ __global__ void __launch_bounds__(4 * kBlk)
   if (UPD)
     fused_move(R, t.x, i, x, y, z, fx, fy, fz, nrm[lane], nrm[kBlk + lane], nrm[2 * kBlk + lane],
                px, py, pz, mv);
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-list walk (BD_WALK_LIST, no reference counterpart).  Each
+// particle keeps the ids of every particle of either grid within rc + skin,
+// rebuilt for the whole replica on the first force evaluation after a move
+// took some particle more than half the skin from its position at the last
+// build (Verlet's criterion).  A step then tests only the listed candidates.
+// The reference order of the pair terms is recovered exactly: a partner
+// inside the cutoff gets the key (pass, offset of its cell from the
+// particle's cell in the walk's stencil order, id), which is its position in
+// the cell walk (cells in (di, dj, dk) order, ids ascending within a cell),
+// and the partners are evaluated in key order after an insertion sort in
+// LDS.  For a pair inside the cutoff the minimum image is the image of the
+// cell walk (grids of at least 2 nrange + 3 cells), so the separations and
+// the terms are the walk's to the bit.  A particle whose list or sorted
+// partners would overflow walks its cells instead.
+// ---------------------------------------------------------------------------
+constexpr int kListCap = 48;  // listed candidates per particle
+constexpr int kHitCap = 32;   // partners inside the cutoff sorted per particle
+
+struct ListArgs {
+  unsigned int* nlist;   // [kListCap][n_total]: replica-local id | pass bit
+  int* ncount;           // listed candidates per particle, -1: walk the cells
+  double *rbx, *rby, *rbz;  // raw positions at the build
+  const double *rx, *ry, *rz;
+  const int* lflag;      // rebuild flags, [2 R]
+  const int* list_ok;    // [R]: the grids hold the list stencil
+  int n_total;
+  double skin;
+};
+
+__device__ inline double min_image(double e, double L, double halfL) {
+  return e > halfL ? -L : (e < -halfL ? L : 0.0);
+}
+
+// Candidate list of replica-local particle i (global index p) at (x, y, z):
+// every particle of either grid closer than rc + skin, from the packed rows.
+// Returns the count, or -1 when it exceeds kListCap.
+__device__ inline int build_list(const RepDev& R, int i, int p, double x, double y, double z,
+                                 const int* __restrict__ rs, const double4* __restrict__ cpos,
+                                 const ListArgs& la) {
+  const int g = i < R.nb ? 0 : 1;
+  const double L = R.side, halfL = 0.5 * R.side;
+  int cnt = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    const int gg = pass == 0 ? g : 1 - g;
+    const double rl = sqrt(pass == 0 ? R.rc2_same[g] : R.rc2_bs) + la.skin;
+    const double rl2 = rl * rl;
+    const int n = R.ncell[gg];
+    const int nr = (int)ceil(rl / R.cell_side[gg]);
+    int h[3];
+    if (!cell_coords(R, gg, x, y, z, h[0], h[1], h[2])) return -1;
+    const int* rs_g = rs + R.rs_off[gg];
+    const double4* cp = cpos + R.tag_off[gg];
+    const double self = pass == 0 ? (double)i : -1.0;
+    const unsigned int pb = pass == 0 ? 0u : kPassBit;
+    for (int di = -nr; di <= nr; di++) {
+      int a = h[0] + di;
+      a = a < 0 ? a + n : (a >= n ? a - n : a);
+      for (int dj = -nr; dj <= nr; dj++) {
+        int b = h[1] + dj;
+        b = b < 0 ? b + n : (b >= n ? b - n : b);
+        const int* rrow = rs_g + (a + n * b) * (n + 1);
+        for (int dk = -nr; dk <= nr; dk++) {
+          int c = h[2] + dk;
+          c = c < 0 ? c + n : (c >= n ? c - n : c);
+          const int s1 = rrow[c + 1];
+          for (int sl = rrow[c]; sl < s1; sl++) {
+            const double4 q = cp[sl];
+            if (q.w == self) continue;
+            double dx = q.x - x, dy = q.y - y, dz = q.z - z;
+            dx += min_image(dx, L, halfL);
+            dy += min_image(dy, L, halfL);
+            dz += min_image(dz, L, halfL);
+            if (dx * dx + dy * dy + dz * dz < rl2) {
+              if (cnt < kListCap) la.nlist[(size_t)cnt * la.n_total + p] = (unsigned int)q.w | pb;
+              cnt++;
+            }
+          }
+        }
+      }
+    }
+  }
+  return cnt <= kListCap ? cnt : -1;
+}
+
+// Forces of every particle from its list.  The candidates' positions are
+// read from px/py/pz by id, so the update cannot be fused into this kernel
+// (a moved particle would be seen by lanes still testing it).
+template <int POT>
+__global__ void __launch_bounds__(kBlk)
+    k_forces_list(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+                  const double* __restrict__ px, const double* __restrict__ py,
+                  const double* __restrict__ pz, const int* __restrict__ rs,
+                  const double4* __restrict__ cpos, double* __restrict__ ffx,
+                  double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
+                  double* __restrict__ pot, const uint64_t* __restrict__ draws, ListArgs la) {
+  __shared__ unsigned int keys[kHitCap * kBlk];
+  const int4 t = tbl[blockIdx.x];
+  if (t.x < 0) return;
+  const int i = t.y + (int)threadIdx.x;
+  if (i >= t.z) return;
+  const RepDev& R = reps[t.x];
+  const int p = R.p_off + i;
+  const int g = i < R.nb ? 0 : 1;
+  const double x = px[p], y = py[p], z = pz[p];
+  const double L = R.side, halfL = 0.5 * R.side;
+  double fx = 0.0, fy = 0.0, fz = 0.0, v = 0.0, u = 0.0;
+  bool walk = !R.fast || !la.list_ok[t.x];
+  if (!walk) {
+    int cnt;
+    if (la.lflag[2 * t.x + list_parity(R, draws[t.x], 0)]) {
+      cnt = build_list(R, i, p, x, y, z, rs, cpos, la);
+      la.ncount[p] = cnt;
+      la.rbx[p] = la.rx[p];
+      la.rby[p] = la.ry[p];
+      la.rbz[p] = la.rz[p];
+    } else {
+      cnt = la.ncount[p];
+    }
+    int h[2][3];
+    if (cnt < 0 || !cell_coords(R, g, x, y, z, h[0][0], h[0][1], h[0][2]) ||
+        !cell_coords(R, 1 - g, x, y, z, h[1][0], h[1][1], h[1][2]))
+      walk = true;
+    unsigned int* kc = keys + threadIdx.x;
+    int nh = 0;
+    for (int c = 0; c < cnt && !walk; c++) {
+      const unsigned int e = la.nlist[(size_t)c * la.n_total + p];
+      const int pass = (int)(e >> 31);
+      const int j = (int)(e & ~kPassBit);
+      const int q = R.p_off + j;
+      const double qx = px[q], qy = py[q], qz = pz[q];
+      const double ex = qx - x, ey = qy - y, ez = qz - z;
+      const double dx = ex + min_image(ex, L, halfL);
+      const double dy = ey + min_image(ey, L, halfL);
+      const double dz = ez + min_image(ez, L, halfL);
+      const double dist2 = dx * dx + dy * dy + dz * dz;
+      if (dist2 < (pass ? R.rc2_bs : R.rc2_same[g])) {
+        // the partner's place in the cell walk: stencil cell, then id
+        const int gg = pass ? 1 - g : g;
+        const int nr = pass ? R.nrange_cross[g] : R.nrange_same[g];
+        const int n = R.ncell[gg];
+        int cj[3] = {0, 0, 0};
+        cell_coords(R, gg, qx, qy, qz, cj[0], cj[1], cj[2]);
+        int d[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          const int o = cj[a] - h[pass][a];
+          d[a] = (o > nr ? o - n : (o < -nr ? o + n : o)) + nr;
+        }
+        const unsigned int key = ((unsigned int)pass << 31) |
+                                 ((unsigned int)((d[0] * (2 * nr + 1) + d[1]) * (2 * nr + 1) +
+                                                 d[2]) << 24) |
+                                 (unsigned int)j;
+        if (nh == kHitCap) {
+          walk = true;
+          break;
+        }
+        int pos = nh;
+        while (pos > 0 && kc[(pos - 1) * kBlk] > key) {
+          kc[pos * kBlk] = kc[(pos - 1) * kBlk];
+          pos--;
+        }
+        kc[pos * kBlk] = key;
+        nh++;
+      }
+    }
+    if (!walk) {
+      for (int e = 0; e < nh; e++) {
+        const unsigned int key = kc[e * kBlk];
+        const int pass = (int)(key >> 31);
+        const int q = R.p_off + (int)(key & 0xffffffu);
+        const double ex = px[q] - x, ey = py[q] - y, ez = pz[q] - z;
+        const double dx = ex + min_image(ex, L, halfL);
+        const double dy = ey + min_image(ey, L, halfL);
+        const double dz = ez + min_image(ez, L, halfL);
+        const double dist2 = dx * dx + dy * dy + dz * dz;
+        pair_add<POT>(R, pass ? R.sigma_bs : R.sigma[g], dist2, dx, dy, dz, fx, fy, fz, v, u);
+      }
+    }
+  }
+  if (walk) {
+    fx = fy = fz = v = u = 0.0;
+    forces_global<POT, false>(R, i, x, y, z, rs, cpos, fx, fy, fz, v, u);
+  }
+  ffx[p] = fx;
+  ffy[p] = fy;
+  ffz[p] = fz;
+  vir[p] = v;
+  pot[p] = u;
 }
 
 // ---------------------------------------------------------------------------
@@ -2258,6 +2486,13 @@ struct bd_sys {
   // kernel timing: when set, launch() has the runtime stamp these at the
   // kernel's start and end
   hipEvent_t tev[2] = {nullptr, nullptr};
+  // neighbour lists of the list walk (allocated on first use)
+  double skin = 0.3;
+  unsigned int* nlist = nullptr;
+  int* ncount = nullptr;
+  double *rbx = nullptr, *rby = nullptr, *rbz = nullptr;
+  int* lflag = nullptr;    // [2 R] rebuild flags
+  int* list_ok = nullptr;  // [R]
   bool defer_ok = true;  // every grid's slot index fits the 26-bit record
   std::vector<int4> tbl_split_h;
   int4* tbl_split = nullptr;
@@ -2295,24 +2530,93 @@ int launch_bin(bd_sys* s) {
   return BD_OK;
 }
 
+bool list_on(const bd_sys* s) {
+  return s->walk == BD_WALK_LIST && s->brick == 0 && s->order == BD_ORDER_REFERENCE;
+}
+
+// The moves' list bookkeeping: live only while the list walk is the force path.
+ListMove list_move(const bd_sys* s) {
+  if (!list_on(s)) return ListMove{nullptr, nullptr, nullptr, nullptr, 0.0};
+  return ListMove{s->rbx, s->rby, s->rbz, s->lflag, 0.25 * s->skin * s->skin};
+}
+
 void launch_update(bd_sys* s, int do_bin) {
   const int nb = (int)s->tbl_lane_h.size();
-  launch(s, k_update_bin, nb, kBlk, s->reps_d, s->tbl_lane, s->px, s->py, s->pz, s->rx,
-                                           s->ry, s->rz, s->fx, s->fy, s->fz, s->ring, s->lcg,
-                                           s->draws, s->step, s->lanes_total, s->nocup, s->ins,
-                                           s->flags, do_bin);
+  launch(s, k_update_bin, nb, kBlk, s->reps_d, s->tbl_lane, s->px, s->py, s->pz, s->rx, s->ry,
+         s->rz, s->fx, s->fy, s->fz, s->ring, s->lcg, s->draws, s->step, s->lanes_total,
+         s->nocup, s->ins, s->flags, do_bin, list_move(s));
 }
 
 void launch_cells(bd_sys* s, int advance) {
   const int nb = (int)s->tbl_cell_h.size();
-  launch(s, k_cells, nb, kBlk, s->reps_d, s->tbl_cell, s->px, s->py, s->pz, s->nocup,
-                                      s->cnt, s->ins, s->rs, s->cpos, s->flags, s->step,
-                                      s->draws, advance);
+  launch(s, k_cells, nb, kBlk, s->reps_d, s->tbl_cell, s->px, s->py, s->pz, s->nocup, s->cnt,
+         s->ins, s->rs, s->cpos, s->flags, s->step, s->draws, advance,
+         list_on(s) ? s->lflag : nullptr);
 }
 
 MoveArgs move_args(bd_sys* s) {
-  return MoveArgs{s->ring, s->lcg,  s->draws, s->step,  s->lanes_total, s->rx,
-                  s->ry,   s->rz,   s->nocup, s->ins,   s->flags};
+  return MoveArgs{s->ring, s->lcg,   s->draws, s->step,  s->lanes_total, s->rx,
+                  s->ry,   s->rz,    s->nocup, s->ins,   s->flags,       list_move(s)};
+}
+
+void launch_list(bd_sys* s, const int4* tbl, int nblk) {
+  const ListArgs la{s->nlist, s->ncount, s->rbx, s->rby,   s->rbz,        s->rx,
+                    s->ry,    s->rz,     s->lflag, s->list_ok, s->n_total, s->skin};
+  switch (s->pot_kind) {
+    case BD_POT_WCA:
+      launch(s, k_forces_list<BD_POT_WCA>, nblk, kBlk, s->reps_d, tbl, s->px, s->py, s->pz, s->rs,
+             s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->draws, la);
+      break;
+    case BD_POT_WCA_MODIFIED:
+      launch(s, k_forces_list<BD_POT_WCA_MODIFIED>, nblk, kBlk, s->reps_d, tbl, s->px, s->py,
+             s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->draws, la);
+      break;
+    default:
+      launch(s, k_forces_list<BD_POT_HARMONIC>, nblk, kBlk, s->reps_d, tbl, s->px, s->py, s->pz,
+             s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->draws, la);
+      break;
+  }
+}
+
+// Every list is stale: the next force evaluation of each replica rebuilds.
+int list_invalidate(bd_sys* s) {
+  if (!s->lflag) return BD_OK;
+  std::vector<int> ones(2 * (size_t)s->R, 1);
+  HIP_TRY(hipMemcpyAsync(s->lflag, ones.data(), sizeof(int) * ones.size(), hipMemcpyHostToDevice,
+                         s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return BD_OK;
+}
+
+// Allocates the lists and decides per replica whether its grids hold the
+// list stencil: ceil((rc + skin) / cell side) cells each way, without a cell
+// met twice (2 nr + 1 <= ncell), rc + skin below half the box, ids below 2^24.
+int list_setup(bd_sys* s) {
+  if (!s->nlist) {
+    int rc = dev_alloc(&s->nlist, (size_t)kListCap * s->n_total);
+    if (rc == BD_OK) rc = dev_alloc(&s->ncount, (size_t)s->n_total);
+    if (rc == BD_OK) rc = dev_alloc(&s->rbx, (size_t)s->n_total);
+    if (rc == BD_OK) rc = dev_alloc(&s->rby, (size_t)s->n_total);
+    if (rc == BD_OK) rc = dev_alloc(&s->rbz, (size_t)s->n_total);
+    if (rc == BD_OK) rc = dev_alloc(&s->lflag, 2 * (size_t)s->R);
+    if (rc == BD_OK) rc = dev_alloc(&s->list_ok, (size_t)s->R);
+    if (rc != BD_OK) return rc;
+  }
+  std::vector<int> ok(s->R, 0);
+  for (int r = 0; r < s->R; r++) {
+    const RepDev& R = s->reps[r];
+    bool good = R.fast && R.n < (1 << 24);
+    for (int g = 0; g < 2 && good; g++)
+      for (int pass = 0; pass < 2 && good; pass++) {
+        const int gg = pass == 0 ? g : 1 - g;
+        const double rl = std::sqrt(pass == 0 ? R.rc2_same[g] : R.rc2_bs) + s->skin;
+        const int nr = (int)std::ceil(rl / R.cell_side[gg]);
+        good = 2 * nr + 1 <= R.ncell[gg] && rl < 0.5 * R.side;
+      }
+    ok[r] = good ? 1 : 0;
+  }
+  HIP_TRY(hipMemcpy(s->list_ok, ok.data(), sizeof(int) * ok.size(), hipMemcpyHostToDevice));
+  return list_invalidate(s);
 }
 
 // UPD: the force kernel also performs the next step's update (see k_forces).
@@ -2428,6 +2732,8 @@ template <bool NEIGH>
 void launch_forces_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl, int nblk) {
   if (NEIGH)
     launch_forces_k<true, 0, 0, false>(s, ncounts, nlist, maxn, tbl, nblk);
+  else if (s->walk == BD_WALK_LIST)
+    launch_list(s, tbl, nblk);
   else if (s->walk >= 1)
     launch_slab_forces<false>(s, tbl, nblk);
   else
@@ -2486,7 +2792,8 @@ void launch_split_t(bd_sys* s) {
 // loses more to the added registers than the launch it saves (2.88e9 vs
 // 3.02e9 on 256 replicas).
 bool fusable(const bd_sys* s) {
-  if (s->fuse == 0 || s->order != BD_ORDER_REFERENCE || s->brick != 0 || s->walk < 1)
+  if (s->fuse == 0 || s->order != BD_ORDER_REFERENCE || s->brick != 0 || s->walk < 1 ||
+      s->walk == BD_WALK_LIST)
     return false;
   if (s->fuse == 2 && (s->pass_split == 0 || s->defer == 0)) return false;
   for (int r = 0; r < s->R; r++)
@@ -2651,6 +2958,9 @@ void free_all(bd_sys* s) {
   if (s->tbl_brick) (void)hipFree(s->tbl_brick);
   if (s->tbl_split) (void)hipFree(s->tbl_split);
   if (s->origins) (void)hipFree(s->origins);
+  void* lists[] = {s->nlist, s->ncount, s->rbx, s->rby, s->rbz, s->lflag, s->list_ok};
+  for (void* p : lists)
+    if (p) (void)hipFree(p);
   void* ptrs[] = {s->tbl_part, s->tbl_lane, s->tbl_cell, s->reps_d, s->px,   s->py,
                   s->pz,       s->rx,       s->ry,       s->rz,     s->ix,   s->iy,
                   s->iz,       s->fx,       s->fy,       s->fz,     s->vir,  s->pot,
@@ -3050,6 +3360,7 @@ int bd_set_positions(bd_sys* s, int32_t replica, const double* xyz) {
   if (rc == BD_OK) {
     s->have_pos[replica] = true;
     s->located = false;
+    rc = list_invalidate(s);
   }
   return rc;
 }
@@ -3263,7 +3574,9 @@ int bd_step(bd_sys* s, int32_t nsteps) {
     }
     HIP_TRY(hipGetLastError());
     s->ni += nsteps;
-    return check_flags(s);
+    rc = check_flags(s);
+    // the persistent kernel keeps no neighbour lists
+    return rc == BD_OK ? list_invalidate(s) : rc;
   }
   const int G = s->graph_mode;
   int left = nsteps;
@@ -3326,7 +3639,8 @@ int bd_set_brick(bd_sys* s, int32_t brick) {
     s->graph = nullptr;
     s->graph_steps = 0;
   }
-  return setup_bricks(s, brick);
+  const int rc = setup_bricks(s, brick);
+  return rc == BD_OK ? list_invalidate(s) : rc;
 }
 
 int bd_set_force_order(bd_sys* s, int32_t order) {
@@ -3354,7 +3668,7 @@ int bd_set_force_order(bd_sys* s, int32_t order) {
     s->tbl_split_h = tbl;
   }
   s->order = order;
-  return BD_OK;
+  return list_invalidate(s);
 }
 
 // Autotuner: average duration of the force kernel for every brick shape the
@@ -3414,7 +3728,9 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
     s->graph_steps = 0;
   }
   // kernel shapes: the row, slab and chunked slab walks at three deferral
-  // depths, the LDS bricks
+  // depths, the LDS bricks.  The list walk is not among them: forces-only
+  // launches between rebuilds would not see its rebuild cost, which decides
+  // it (measured slower on the workloads here).
   struct Shape {
     int brick, defer, walk, split;
   };
@@ -3462,7 +3778,8 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
   s->defer = best.defer;
   s->walk = best.walk;
   s->pass_split = best.split;
-  rc = bd_set_brick(s, best.brick);
+  rc = list_invalidate(s);
+  if (rc == BD_OK) rc = bd_set_brick(s, best.brick);
   if (rc != BD_OK) {
     s->defer = keep.defer;
     s->walk = keep.walk;
@@ -3483,15 +3800,38 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
 int bd_set_walk(bd_sys* s, int32_t walk) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (walk != BD_WALK_ROWS && walk != BD_WALK_SLABS && walk != BD_WALK_CHUNK2 &&
-      walk != BD_WALK_CHUNK4)
-    return fail(BD_E_ARG, "walk must be BD_WALK_ROWS, _SLABS, _CHUNK2 or _CHUNK4");
+      walk != BD_WALK_CHUNK4 && walk != BD_WALK_LIST)
+    return fail(BD_E_ARG, "walk must be BD_WALK_ROWS, _SLABS, _CHUNK2, _CHUNK4 or _LIST");
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (s->graph) {
     (void)hipGraphExecDestroy(s->graph);
     s->graph = nullptr;
     s->graph_steps = 0;
   }
+  if (walk == BD_WALK_LIST) {
+    const int rc = list_setup(s);
+    if (rc != BD_OK) return rc;
+  }
   s->walk = walk;
+  return list_invalidate(s);
+}
+
+int bd_set_list_skin(bd_sys* s, double skin) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (!(skin > 0.0 && skin <= 2.0)) return fail(BD_E_ARG, "skin must lie in (0, 2]");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->graph) {
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_steps = 0;
+  }
+  s->skin = skin;
+  return s->nlist ? list_setup(s) : BD_OK;
+}
+
+int bd_get_list_skin(const bd_sys* s, double* skin) {
+  if (!s || !skin) return fail(BD_E_ARG, "null argument");
+  *skin = s->skin;
   return BD_OK;
 }
 
@@ -3637,7 +3977,7 @@ int bd_checkpoint_load(bd_sys* s, const char* path) {
   HIP_TRY(hipMemcpy(s->step, &s->ni, sizeof(int64_t), hipMemcpyHostToDevice));
   for (int r = 0; r < s->R; r++) s->have_pos[r] = true;
   s->located = false;  // cells are rebuilt (and the stored forces recomputed) on the next step
-  return BD_OK;
+  return list_invalidate(s);
 }
 
 // Kernel durations for the timing entry points: each launch between

@@ -207,12 +207,22 @@ class Engine:
         return v.value
 
     def set_walk(self, walk):
-        """Force walk: abi.BD_WALK_ROWS (0), _SLABS (1), _CHUNK2 (2) or _CHUNK4 (3)."""
+        """Force walk: abi.BD_WALK_ROWS (0), _SLABS (1), _CHUNK2 (2), _CHUNK4 (3)
+        or _LIST (4, neighbour lists)."""
         check(self.lib.bd_set_walk(self.h, int(walk)))
 
     def walk(self):
         v = C.c_int32()
         check(self.lib.bd_get_walk(self.h, C.byref(v)))
+        return v.value
+
+    def set_list_skin(self, skin):
+        """Skin of the neighbour lists (BD_WALK_LIST), default 0.3."""
+        check(self.lib.bd_set_list_skin(self.h, float(skin)))
+
+    def list_skin(self):
+        v = C.c_double()
+        check(self.lib.bd_get_list_skin(self.h, C.byref(v)))
         return v.value
 
     def set_pass_split(self, split):

@@ -176,9 +176,31 @@ int bd_get_defer_depth(const bd_sys* s, int32_t* depth);
  * bounds of a slab loaded together and the candidates loaded 2 / 4 at a
  * time (fewer exposed memory latencies).  Same pairs, order and arithmetic:
  * bit-identical. */
-enum { BD_WALK_ROWS = 0, BD_WALK_SLABS = 1, BD_WALK_CHUNK2 = 2, BD_WALK_CHUNK4 = 3 };
+enum {
+  BD_WALK_ROWS = 0,
+  BD_WALK_SLABS = 1,
+  BD_WALK_CHUNK2 = 2,
+  BD_WALK_CHUNK4 = 3,
+  BD_WALK_LIST = 4
+};
 int bd_set_walk(bd_sys* s, int32_t walk);
 int bd_get_walk(const bd_sys* s, int32_t* walk);
+
+/* Neighbour lists (BD_WALK_LIST, no reference counterpart): each particle
+ * lists every particle within rc + skin; the lists of a replica are rebuilt
+ * from the cells on the first force evaluation after a move took some
+ * particle more than skin/2 from its position at the last build.  Between
+ * rebuilds a step tests only the listed candidates, and sorts the partners
+ * inside the cutoff into the cell walk's order (get_forces.cu:21-558, cells
+ * in (di, dj, dk) order, ids ascending) before adding their terms:
+ * bit-identical to the cell walks.  Replicas whose grids are too small for
+ * the list stencil, and particles with more than 48 candidates or 32
+ * partners, walk their cells.  The update runs as its own kernel with this
+ * walk, and bd_autotune does not choose it (set it explicitly).
+ * bd_set_list_skin: 0 < skin <= 2 (default 0.3, in the unit of sigma);
+ * BD_E_ARG otherwise. */
+int bd_set_list_skin(bd_sys* s, double skin);
+int bd_get_list_skin(const bd_sys* s, double* skin);
 
 /* Stencil passes over wavefronts (no reference counterpart): 0 (default)
  * walks both passes of a particle (main.cu:576-585) in one lane; 1 walks the

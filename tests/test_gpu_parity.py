@@ -157,7 +157,7 @@ def test_every_brick_shape_bit_identical(config1_positions, pot):
     for bad in (-1, 8, 64):
         with pytest.raises(BDError):
             e.set_defer_depth(bad)
-    for bad in (-1, 4):
+    for bad in (-1, 5):
         with pytest.raises(BDError):
             e.set_walk(bad)
     for bad in (-1, 3):
@@ -188,6 +188,9 @@ def test_pass_split_column_overflow(config1_positions):
     e = Engine(p)
     e.set_positions(x)
     e.cell_locate()
+    e.set_walk(abi.BD_WALK_LIST)  # 40 + listed candidates: the particle walks its cells
+    e.compute_forces()
+    assert all(np.array_equal(a, b) for a, b in zip(e.forces(), (fo, vo, uo)))
     for walk in (abi.BD_WALK_SLABS, abi.BD_WALK_CHUNK2, abi.BD_WALK_CHUNK4):
         for depth in (16, 32):
             for split in (1, 2):
@@ -236,6 +239,46 @@ def test_fused_update_bit_identical(config1_positions, split):
     e.close()
 
 
+def test_list_walk_bit_identical(config1_positions):
+    """Neighbour lists against the slab walk: the first forces equal the
+    oracle's, and 400 steps at T=1 (lists rebuilt many times; a 0.05 skin
+    rebuilds almost every step) leave positions, raw positions, forces and the
+    rannew64 lanes identical.  The other walks' test covers the sort: a
+    partner sorted into the wrong place changes the sums' last bits."""
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, T0=1.0, Tf=1.0, seed=21)
+    o = orc.System(p)
+    o.set_positions(x)
+    o.cell_locate()
+    o.compute_forces()
+    fo = o.forces()
+    o.close()
+    runs = []
+    for walk, skin in ((abi.BD_WALK_SLABS, None), (abi.BD_WALK_LIST, None),
+                       (abi.BD_WALK_LIST, 0.05)):
+        e = Engine(p)
+        e.set_walk(walk)
+        assert e.walk() == walk
+        if skin is not None:
+            e.set_list_skin(skin)
+            assert e.list_skin() == skin
+        e.set_positions(x)
+        e.cell_locate()
+        e.compute_forces()
+        assert all(np.array_equal(a, b) for a, b in zip(e.forces(), fo)), (walk, skin)
+        e.step(400)
+        runs.append([e.positions(), e.raw_positions(), *e.forces(), *e.rng_lane(7)])
+        e.close()
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert np.array_equal(np.asarray(a), np.asarray(b))
+    e = Engine(p)
+    for bad in (0.0, -0.1, 2.5):
+        with pytest.raises(BDError):
+            e.set_list_skin(bad)
+    e.close()
+
+
 @pytest.fixture(scope="module")
 def plain_positions():
     """N=32768 (16384 + 16384) at phi=0.58, no overlaps: the bench workload."""
@@ -259,8 +302,8 @@ def test_forces_bit_exact_at_bench_size(plain_positions):
               (0, 32, 1, 0), (0, 0, 2, 0), (0, 16, 2, 0), (0, 32, 2, 0), (0, 0, 3, 0),
               (0, 16, 3, 0), (0, 32, 3, 0), (0, 16, 1, 1), (0, 32, 1, 1), (0, 16, 2, 1),
               (0, 32, 2, 1), (0, 16, 3, 1), (0, 32, 3, 1), (0, 16, 1, 2), (0, 32, 1, 2),
-              (0, 16, 2, 2), (0, 32, 2, 2), (0, 16, 3, 2), (0, 32, 3, 2), (1, 16, 0, 0),
-              (2, 16, 0, 0)]
+              (0, 16, 2, 2), (0, 32, 2, 2), (0, 16, 3, 2), (0, 32, 3, 2), (0, 0, 4, 0),
+              (1, 16, 0, 0), (2, 16, 0, 0)]
     for b, depth, walk, split in shapes:
         e.set_brick(b)
         e.set_defer_depth(depth)
