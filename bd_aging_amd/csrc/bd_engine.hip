@@ -1349,9 +1349,12 @@ __device__ __forceinline__ void k_forces_block(
 // every particle from those forces (the binning into the insertion lists
 // included), which k_update_bin would otherwise do in a launch of its own.
 // The fused slab walk at depth <= 16 is held to the unfused kernel's four
-// wavefronts per SIMD (the move alone would take it two registers past 128).
+// wavefronts per SIMD (the move alone would take it two registers past 128);
+// the 4-candidate chunked walk to three (batch 347 instead of 407 us).
 template <int POT, bool NEIGH, int DEPTH, int WALK, bool UPD>
-__global__ void __launch_bounds__(kBlk, (UPD && WALK == 1 && DEPTH <= 16) ? 4 : 1)
+__global__ void __launch_bounds__(kBlk, (UPD && WALK == 1 && DEPTH <= 16) ? 4
+                                       : (WALK == 3 && DEPTH <= 16)       ? 3
+                                                                          : 1)
     k_forces(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
              double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
              const int* __restrict__ rs, const double4* __restrict__ cpos,

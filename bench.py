@@ -38,8 +38,8 @@ FLOP_PAIR = 22             # FP64 operations per ordered pair in the g(r) scan
 # profiles/k_forces_traffic_pmc.txt, scaled by a copy kernel of known size:
 # plain N=32768 with the four-wavefront chunked walk (forces only, and forces
 # with the fused update), batch with the slab walk (both at deferral depth 16)
-TRAFFIC_FORCES_PLAIN = 6.60e6
-TRAFFIC_FUSED_PLAIN = 1.465e7
+TRAFFIC_FORCES_PLAIN = 6.73e6
+TRAFFIC_FUSED_PLAIN = 1.484e7
 TRAFFIC_FORCES_BATCH = 1.489e8
 
 
