@@ -14,7 +14,11 @@
 //                particles taken in ascending particle order so each entry
 //                is the same sequence of IEEE additions as the serial loop
 //                (Contracting_Vector3D.cu:135-148).
-//   k_cv3d_atb   one thread per (axis, bin), same order (:152-156).
+//   k_cv3d_hits_wave  the same rows with one wavefront per particle (bins <=
+//                2474), up to four per workgroup sharing LDS tiles of candidates: 64
+//                candidates tested per round, the round's hits added to an LDS
+//                row in ascending j.
+//   (A^T b is the force column of the augmented tiles, :152-156.)
 // The solve (bins^3) runs on the host: cyclic Jacobi eigen-decomposition of
 // the symmetric matrix, singular values |lambda| truncated at 1e-6 of the
 // largest (:194-282).
@@ -106,51 +110,165 @@ __global__ void __launch_bounds__(kHitThreads)
   }
 }
 
-// One 16x16 tile (i-tile >= j-tile) of A^T A for axis blockIdx.z.
+// The hit rows with one wavefront per particle m (bins <= 2474): the
+// lanes test 64 candidates j per round, in ascending order, and the round's
+// hits are added to the particle's row in LDS one after the other in lane
+// order, so every bin receives the serial loop's additions in the serial
+// order (ascending j).  Each separation and unit vector is computed with
+// the operations of k_cv3d_hits.
+// LDS of the wavefront kernel: a 256-candidate tile and one row per wavefront,
+// within the 64 KB a workgroup gets without an opt-in
+constexpr size_t kHitLds = 64 * 1024;
+__host__ inline int hit_waves(int bins) {
+  const size_t row = 3 * (size_t)bins * sizeof(double), tile = 768 * sizeof(double);
+  const int w = row > 0 && kHitLds > tile ? (int)((kHitLds - tile) / row) : 0;
+  return w < 4 ? w : 4;
+}
+
+__device__ inline double lane_value(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+__global__ void __launch_bounds__(4 * 64)
+    k_cv3d_hits_wave(const double* __restrict__ px, const double* __restrict__ py,
+                     const double* __restrict__ pz, int n, double side, int periodic,
+                     double range_sq, double bin_size_inv, int bins, double* __restrict__ hit) {
+  // one particle per wavefront, the workgroup's wavefronts sharing each tile
+  // of 256 candidates staged in LDS; then their rows (3 x bins each)
+  extern __shared__ double sh[];
+  double* tx = sh;
+  double* ty = sh + 256;
+  double* tz = sh + 512;
+  const int waves = (int)blockDim.x / 64;
+  const int w = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+  const int m = (int)blockIdx.x * waves + w;
+  const bool active = m < n;
+  double* row = sh + 768 + (size_t)w * 3 * bins;
+  for (int k = lane; k < 3 * bins; k += 64) row[k] = 0.0;
+  double xm = 0.0, ym = 0.0, zm = 0.0;
+  if (active) {
+    xm = px[m];
+    ym = py[m];
+    zm = pz[m];
+  }
+  const double side_inv = periodic ? 1.0 / side : 0.0;
+  for (int jt = 0; jt < n; jt += 256) {
+    __syncthreads();
+    for (int k = (int)threadIdx.x; k < 256; k += (int)blockDim.x) {
+      const int jl = jt + k;
+      if (jl < n) {
+        tx[k] = px[jl];
+        ty[k] = py[jl];
+        tz[k] = pz[jl];
+      }
+    }
+    __syncthreads();
+    if (!active) continue;
+    for (int r = 0; r < 4; r++) {
+      const int t = r * 64 + lane;
+      const int j = jt + t;
+      bool h = false;
+      int nb = 0;
+      double ux = 0.0, uy = 0.0, uz = 0.0;
+      if (j < n && j != m) {
+        double dx = xm - tx[t];
+        double dy = ym - ty[t];
+        double dz = zm - tz[t];
+        if (periodic) {
+          dx -= side * floor(side_inv * dx + 0.5);
+          dy -= side * floor(side_inv * dy + 0.5);
+          dz -= side * floor(side_inv * dz + 0.5);
+        }
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < range_sq) {
+          const double dist = sqrt(d2);
+          nb = (int)(bin_size_inv * dist);
+          if (nb >= bins) nb = bins - 1;
+          ux = dx / dist;
+          uy = dy / dist;
+          uz = dz / dist;
+          h = true;
+        }
+      }
+      unsigned long long mask = __ballot(h);
+      while (mask) {
+        const int l = __ffsll(mask) - 1;
+        mask &= mask - 1;
+        const int b = __builtin_amdgcn_readlane(nb, l);
+        const double vx = lane_value(ux, l), vy = lane_value(uy, l), vz = lane_value(uz, l);
+        if (lane < 3) row[lane * bins + b] += lane == 0 ? vx : (lane == 1 ? vy : vz);
+      }
+    }
+  }
+  if (!active) return;
+  double* out = hit + (size_t)m * 3 * bins;
+  for (int k = lane; k < 3 * bins; k += 64) out[k] = row[k];
+}
+
+// One 16x16 tile (i-tile >= j-tile) of the normal equations of axis
+// blockIdx.z over the augmented columns 0..bins: column bins is the force
+// component, so the entries (bins, j) are A^T b (Contracting_Vector3D.cu:
+// 152-156) and the others A^T A (:135-148).  Each entry sums its products
+// over particles in ascending order.  The particles are taken 32 at a time
+// through double-buffered LDS, the next 32 rows loaded while these are used.
+constexpr int kChunk = 32;
+
+__device__ inline double aug_value(const double* __restrict__ hit, const double* __restrict__ f,
+                                   int m, int a, int c, int bins) {
+  return c < bins ? hit[((size_t)m * 3 + a) * bins + c] : (c == bins ? f[m] : 0.0);
+}
+
 __global__ void __launch_bounds__(kTile * kTile)
-    k_cv3d_ata(const double* __restrict__ hit, int n, int bins, double* __restrict__ ata) {
+    k_cv3d_ata(const double* __restrict__ hit, const double* __restrict__ fx,
+               const double* __restrict__ fy, const double* __restrict__ fz, int n, int bins,
+               double* __restrict__ ata, double* __restrict__ atb) {
   const int ti = (int)blockIdx.y, tj = (int)blockIdx.x;
   if (tj > ti) return;
   const int a = (int)blockIdx.z;
-  __shared__ double hi[kTile][kTile + 1], hj[kTile][kTile + 1];
+  const double* f = a == 0 ? fx : (a == 1 ? fy : fz);
+  __shared__ double hi[2][kChunk][kTile + 1], hj[2][kChunk][kTile + 1];
   const int li = (int)threadIdx.x / kTile, lj = (int)threadIdx.x % kTile;
   const int i = ti * kTile + li, j = tj * kTile + lj;
-  const bool live = i < bins && j < bins && j <= i;
+  const bool live = i <= bins && j < bins && j <= i;
   double* A = ata + (size_t)a * bins * bins;
-  double acc = live ? A[(size_t)i * bins + j] : 0.0;
-  for (int m0 = 0; m0 < n; m0 += kTile) {
-    // load rows m0..m0+15 of the i-tile and j-tile columns
-    const int mm = li, col = lj;
-    const int m = m0 + mm;
-    hi[mm][col] = (m < n && ti * kTile + col < bins)
-                      ? hit[((size_t)m * 3 + a) * bins + ti * kTile + col] : 0.0;
-    hj[mm][col] = (m < n && tj * kTile + col < bins)
-                      ? hit[((size_t)m * 3 + a) * bins + tj * kTile + col] : 0.0;
+  double acc = live ? (i < bins ? A[(size_t)i * bins + j] : atb[(size_t)a * bins + j]) : 0.0;
+  // this thread loads rows li and li + 16 of a chunk, column lj of both tiles
+  double ri[2], rj[2];
+  auto fetch = [&](int m0) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int m = m0 + li + q * kTile;
+      ri[q] = m < n ? aug_value(hit, f, m, a, ti * kTile + lj, bins) : 0.0;
+      rj[q] = m < n ? aug_value(hit, f, m, a, tj * kTile + lj, bins) : 0.0;
+    }
+  };
+  fetch(0);
+  for (int m0 = 0, buf = 0; m0 < n; m0 += kChunk, buf ^= 1) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      hi[buf][li + q * kTile][lj] = ri[q];
+      hj[buf][li + q * kTile][lj] = rj[q];
+    }
     __syncthreads();
-    const int mend = min(kTile, n - m0);
+    if (m0 + kChunk < n) fetch(m0 + kChunk);
+    const int mend = min(kChunk, n - m0);
     if (live)
       for (int k = 0; k < mend; k++) {
-        const double aux = hi[k][li] * hj[k][lj];
+        const double aux = hi[buf][k][li] * hj[buf][k][lj];
         acc += aux;
       }
-    __syncthreads();
   }
   if (live) {
-    A[(size_t)i * bins + j] = acc;
-    A[(size_t)j * bins + i] = acc;
+    if (i < bins) {
+      A[(size_t)i * bins + j] = acc;
+      A[(size_t)j * bins + i] = acc;
+    } else {
+      atb[(size_t)a * bins + j] = acc;
+    }
   }
-}
-
-__global__ void k_cv3d_atb(const double* __restrict__ hit, const double* __restrict__ fx,
-                           const double* __restrict__ fy, const double* __restrict__ fz, int n,
-                           int bins, double* __restrict__ atb) {
-  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (k >= 3 * bins) return;
-  const int a = k / bins, i = k % bins;
-  const double* f = a == 0 ? fx : (a == 1 ? fy : fz);
-  double acc = atb[k];
-  for (int m = 0; m < n; m++) acc += hit[((size_t)m * 3 + a) * bins + i] * f[m];
-  atb[k] = acc;
 }
 
 int accumulate(bd_cv3d* cv, const double* px, const double* py, const double* pz,
@@ -164,17 +282,22 @@ int accumulate(bd_cv3d* cv, const double* px, const double* py, const double* pz
     CV_TRY(hipMalloc((void**)&cv->hit, need * sizeof(double)));
     cv->hit_cap = need;
   }
-  CV_TRY(hipMemsetAsync(cv->hit, 0, need * sizeof(double), st));
+  // the wavefront kernel writes whole rows; the fallback accumulates into zeros
+  if (hit_waves(cv->bins) == 0) CV_TRY(hipMemsetAsync(cv->hit, 0, need * sizeof(double), st));
   const double bin_size = cv->range / (double)cv->bins;
   const double bin_size_inv = 1.0 / bin_size;
-  k_cv3d_hits<<<(n + kHitThreads - 1) / kHitThreads, kHitThreads, 0, st>>>(
-      px, py, pz, n, side, periodic, cv->range * cv->range, bin_size_inv, cv->bins, cv->hit);
+  const int waves = hit_waves(cv->bins);
+  if (waves > 0 && n > 0)
+    k_cv3d_hits_wave<<<(n + waves - 1) / waves, waves * 64,
+                       (768 + 3 * (size_t)waves * cv->bins) * sizeof(double), st>>>(
+        px, py, pz, n, side, periodic, cv->range * cv->range, bin_size_inv, cv->bins, cv->hit);
+  else
+    k_cv3d_hits<<<(n + kHitThreads - 1) / kHitThreads, kHitThreads, 0, st>>>(
+        px, py, pz, n, side, periodic, cv->range * cv->range, bin_size_inv, cv->bins, cv->hit);
   CV_TRY(hipGetLastError());
-  const int nt = (cv->bins + kTile - 1) / kTile;
-  k_cv3d_ata<<<dim3(nt, nt, 3), kTile * kTile, 0, st>>>(cv->hit, n, cv->bins, cv->ata);
-  CV_TRY(hipGetLastError());
-  k_cv3d_atb<<<(3 * cv->bins + 255) / 256, 256, 0, st>>>(cv->hit, fx, fy, fz, n, cv->bins,
-                                                          cv->atb);
+  const int nt = (cv->bins + 1 + kTile - 1) / kTile;
+  k_cv3d_ata<<<dim3(nt, nt, 3), kTile * kTile, 0, st>>>(cv->hit, fx, fy, fz, n, cv->bins,
+                                                        cv->ata, cv->atb);
   CV_TRY(hipGetLastError());
   CV_TRY(hipStreamSynchronize(st));
   cv->samples++;

@@ -2401,6 +2401,13 @@ __global__ void __launch_bounds__(kGderThreads)
 }
 
 // Stand-alone rannew64 lanes for the stream test.
+// Stream copy for the bandwidth probe: b = a over n double4, grid-stride.
+__global__ void __launch_bounds__(256)
+    k_stream_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) b[k] = a[k];
+}
+
 __global__ void k_rng_stream(uint64_t seed, int lanes, int ndraws, uint64_t* __restrict__ ring,
                              uint64_t* __restrict__ out) {
   const int l = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -4010,6 +4017,42 @@ struct KernelTimer {
     *acc += (double)t;
   }
 };
+
+int bd_stream_bandwidth(int64_t bytes, int32_t reps, double* gbs) {
+  if (bytes < (int64_t)(1 << 20) || reps < 1 || !gbs) return fail(BD_E_ARG, "bad arguments");
+  const size_t n = (size_t)bytes / sizeof(double4);
+  double4 *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&a, n * sizeof(double4));
+  if (e == hipSuccess) e = hipMalloc((void**)&b, n * sizeof(double4));
+  if (e == hipSuccess) e = hipStreamCreate(&st);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 0, n * sizeof(double4), st);
+  int dev = 0, ncu = 0;
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int nblk = (int)std::min<size_t>((size_t)ncu * 16, (n + 255) / 256);
+  double total_ms = 0.0;
+  for (int k = 0; k <= reps && e == hipSuccess; k++) {  // launch 0 warms up
+    hipExtLaunchKernelGGL(k_stream_copy, dim3(nblk), dim3(256), 0, st, e0, e1, 0,
+                          (const double4*)a, b, n);
+    e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (k > 0) total_ms += (double)t;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (e != hipSuccess)
+    return fail(BD_E_HIP, std::string("bd_stream_bandwidth: ") + hipGetErrorString(e));
+  *gbs = 2.0 * (double)(n * sizeof(double4)) * reps / (total_ms * 1e-3) / 1e9;
+  return BD_OK;
+}
 
 int bd_kernel_times(bd_sys* s, int32_t nsteps, double* ms) {
   if (!s || !ms || nsteps < 0) return fail(BD_E_ARG, "bad arguments");

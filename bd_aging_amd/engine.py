@@ -331,6 +331,13 @@ def lane_seed(seed, lane):
     return load().bd_lane_seed(int(seed) & ((1 << 64) - 1), int(lane))
 
 
+def stream_bandwidth(nbytes=1 << 30, reps=10):
+    """Achievable HBM bandwidth (GB/s, read + write) of a device copy kernel."""
+    v = C.c_double()
+    check(load().bd_stream_bandwidth(int(nbytes), int(reps), C.byref(v)))
+    return v.value
+
+
 class Contraction:
     """Force contraction (Contracting_Vector3D): add samples, solve, read."""
 

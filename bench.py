@@ -24,7 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from bd_aging_amd.engine import Engine, make_params, box_side  # noqa: E402
+from bd_aging_amd.engine import Contraction, Engine, make_params, box_side, stream_bandwidth  # noqa: E402,E501
 from bd_aging_amd.init import relax_overlaps, relax_batch  # noqa: E402
 
 HBM_PEAK = 8.0e12          # B/s, MI355X
@@ -34,6 +34,9 @@ B_BIN = 36                 # algorithmic bytes per particle, binning pass
 B_UPDATE = 120             # algorithmic bytes per particle, update pass (ring excluded)
 B_STEP = 228               # algorithmic bytes per particle-step (bin 36 + forces 72 + update 120)
 FLOP_PAIR = 22             # FP64 operations per ordered pair in the g(r) scan
+CV3D_BINS, CV3D_RANGE = 400, 5.0   # force-contraction sample: bins to range
+RNG_BYTES = 112            # rannew64 ring and LCG bytes per particle-step (8 x 8 B read,
+                           # 4 x 8 B written, LCG 16 B), beside the update's 120 B
 # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs in
 # profiles/k_forces_traffic_pmc.txt, scaled by a copy kernel of known size:
 # plain N=32768 with the four-wavefront chunked walk (forces only, and forces
@@ -102,10 +105,18 @@ def run_plain(steps, warmup, graph_steps=16, shape=None):
     for _ in range(reps):
         eng.gder_counts(128, 5.0)
     gder_s = (time.perf_counter() - t0) / reps
+    # force contraction (Contracting_Vector3D): one sample of the current state
+    cv = Contraction(CV3D_BINS, CV3D_RANGE)
+    cv.add(eng)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cv.add(eng)
+    cv3d_s = (time.perf_counter() - t0) / reps
+    cv.close()
     eng.close()
     return {"N": nb + ns, "side": side, "seconds": dt, "rate": (nb + ns) * steps / dt,
             "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p, "shape": shape,
-            "gder_s": gder_s, "fused_ms": fused_ms}
+            "gder_s": gder_s, "fused_ms": fused_ms, "cv3d_s": cv3d_s}
 
 
 def run_batch(steps, warmup, graph_steps=16, shape=None):
@@ -157,6 +168,10 @@ def run_stress(steps, warmup, graph_steps=16, shape=None):
     dt = time.perf_counter() - t0
     ms = eng.kernel_times(100) / 100
     geo = eng.geometry()
+    eng.gder_counts(128, 5.0)  # warm-up
+    t0 = time.perf_counter()
+    eng.gder_counts(128, 5.0)
+    gder_s = time.perf_counter() - t0
     eng.close()
     N = nb + ns
     kern = {}
@@ -164,14 +179,45 @@ def run_stress(steps, warmup, graph_steps=16, shape=None):
                        ("forces", B_FORCES, ms[2])):
         gbs = b * N / (m * 1e-3)
         kern[name] = {"ms": m, "bytes_per_particle": b, "GB_s": gbs / 1e9, "hbm_frac": gbs / HBM_PEAK}
+    kern["update"]["rng_bytes_per_particle"] = RNG_BYTES
+    kern["update"]["GB_s_with_rng"] = (B_UPDATE + RNG_BYTES) * N / (ms[0] * 1e-3) / 1e9
     return {"N": N, "side": round(side, 4), "steps": steps, "rate": N * steps / dt,
             "cells": f"{geo.ncell_big}^3 big + {geo.ncell_sml}^3 small",
-            "force_kernel": shape, "kernels": kern}
+            "force_kernel": shape, "kernels": kern,
+            "gder": {"pairs_per_s": N * (N - 1) / gder_s,
+                     "GFLOP_s": FLOP_PAIR * N * (N - 1) / gder_s / 1e9,
+                     "fp64_frac": FLOP_PAIR * N * (N - 1) / gder_s / FP64_PEAK,
+                     "bins": 128, "range": 5.0}}
+
+
+def host_info():
+    model = "?"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable": len(os.sched_getaffinity(0)), "model": model}
 
 
 def cpu_baseline(plain, budget_s=10.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
+    host = host_info()
+    allowed = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, {min(allowed)})  # one pinned core
+    try:
+        out = _cpu_baseline(orc, plain, budget_s)
+    finally:
+        os.sched_setaffinity(0, allowed)
+    out["host"] = host
+    return out
+
+
+def _cpu_baseline(orc, plain, budget_s):
     o = orc.System(plain["params"])
     o.set_positions(plain["x"])
     o.cell_locate()
@@ -186,7 +232,7 @@ def cpu_baseline(plain, budget_s=10.0):
     return {"value": plain["N"] * k / dt, "unit": "particle-steps/s", "cores": 1,
             "kind": "port",
             "sample": f"FP64 oracle, N=32768 phi=0.58, {k} steps after relaxation "
-                      f"({dt:.1f} s, one thread)"}
+                      f"({dt:.1f} s, one thread pinned to one core)"}
 
 
 def _oracle_replica(task):
@@ -217,7 +263,8 @@ def cpu_baseline_batch(pool, workers, batch, replicas=64, steps=150):
     return {"value": work / wall, "unit": "particle-steps/s", "cores": workers, "kind": "port",
             "sample": f"FP64 oracle, first {replicas} replicas of the 256-replica batch "
                       f"(sum N = {sum(t[0] + t[1] for t in tasks)}), {steps} steps each, one "
-                      f"process per replica on {workers} host processes ({wall:.1f} s)"}
+                      f"process per replica on {workers} host processes ({wall:.1f} s)",
+            "host": host_info()}
 
 
 def main():
@@ -278,6 +325,9 @@ def main():
     else:
         f_ms, f_bytes, traffic = plain["kernel_ms"][2], B_FORCES, TRAFFIC_FORCES_PLAIN
     achieved = f_bytes * N / (f_ms * 1e-3)
+    copy_gbs = stream_bandwidth()
+    log(f"[bench] device copy kernel: {copy_gbs:.0f} GB/s")
+    cv3d_flop = FLOP_PAIR * N * (N - 1) + 3 * CV3D_BINS * (CV3D_BINS + 1) * N
     out = {
         "metric": "particle-steps/s at N=32768 phi=0.58 (plain single system)",
         "value": plain["rate"],
@@ -299,7 +349,8 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic, "traffic_unit": "B per launch (rocprofv3)",
                      "bytes_per_particle": f_bytes,
-                     "avg_launch_ms": f_ms},
+                     "avg_launch_ms": f_ms,
+                     "peak_measured_copy": copy_gbs, "frac_of_measured": achieved / 1e9 / copy_gbs},
         "kernel_ms": {"update": plain["kernel_ms"][0], "cells": plain["kernel_ms"][1],
                       "forces": plain["kernel_ms"][2],
                       "forces_with_update": plain["fused_ms"][1] if plain["fused_ms"] else None},
@@ -308,6 +359,10 @@ def main():
                  "GFLOP_s": FLOP_PAIR * N * (N - 1) / plain["gder_s"] / 1e9,
                  "fp64_frac": FLOP_PAIR * N * (N - 1) / plain["gder_s"] / FP64_PEAK,
                  "flop_per_pair": FLOP_PAIR, "bins": 128, "range": 5.0},
+        "cv3d": {"bins": CV3D_BINS, "range": CV3D_RANGE, "ms_per_sample": plain["cv3d_s"] * 1e3,
+                 "GFLOP_s": cv3d_flop / plain["cv3d_s"] / 1e9,
+                 "fp64_frac": cv3d_flop / plain["cv3d_s"] / FP64_PEAK,
+                 "flop_model": "22 per ordered pair + 3 bins (bins + 1) per particle"},
         "cpu_baseline": cpu,
     }
     if stress:

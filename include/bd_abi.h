@@ -272,6 +272,12 @@ int bd_kernel_times(bd_sys* s, int32_t nsteps, double* ms);
  * into ms[0] and of the nsteps - 1 force kernels that also move the particles
  * into ms[1].  BD_E_STATE when the fused update does not apply. */
 int bd_kernel_times_fused(bd_sys* s, int32_t nsteps, double* ms);
+/* HBM bandwidth probe (no reference counterpart): a grid-stride double4 copy
+ * between two device buffers of `bytes` each (>= 1 MiB), timed over `reps`
+ * launches after a warm-up one by runtime-stamped events; *gbs = bytes read
+ * plus bytes written per second, in GB/s (the achievable peak beside the
+ * data-sheet figure). */
+int bd_stream_bandwidth(int64_t bytes, int32_t reps, double* gbs);
 
 /* ---- rannew64 lanes ---- */
 /* State of one lane of a replica: ring[250] (slot order) and the LCG word;

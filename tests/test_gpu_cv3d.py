@@ -43,6 +43,21 @@ def test_normal_equations_bit_exact_host_inputs():
     c.close()
 
 
+def test_many_bins_take_the_thread_per_particle_rows():
+    """Beyond 2474 bins a hit row does not fit 64 KB of LDS and the rows are
+    built one thread per particle: same normal equations as the oracle."""
+    rng = np.random.default_rng(4)
+    bins, rr, n, side = 2700, 3.0, 100, 6.0
+    x = rng.random((n, 3)) * side
+    f = rng.normal(size=(n, 3))
+    c = Contraction(bins, rr)
+    c.add_host(x, f, side)
+    ata, atb = orc.cv3d_accumulate(bins, rr, x, f, side)
+    ga, gb = c.normal()
+    assert np.array_equal(ga, ata) and np.array_equal(gb, atb)
+    c.close()
+
+
 def test_contraction_of_engine_state_matches_oracle():
     side = box_side(400, 400, 1.4, 1.0, 0.45)
     x = relax_overlaps(400, 400, side, seed=8)

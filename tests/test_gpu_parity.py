@@ -659,3 +659,12 @@ def test_split_order_statistics_within_two_sigma(config1_positions):
     a, b = stats[abi.BD_ORDER_REFERENCE], stats[abi.BD_ORDER_SPLIT4]
     err = np.sqrt(a.var(0, ddof=1) / 16 + b.var(0, ddof=1) / 16)
     assert np.all(np.abs(a.mean(0) - b.mean(0)) <= 2.0 * err + 1e-12), (a.mean(0), b.mean(0))
+
+
+def test_stream_bandwidth_probe():
+    """The HBM probe returns a plausible read + write rate and checks its arguments."""
+    from bd_aging_amd.engine import stream_bandwidth
+    gbs = stream_bandwidth(1 << 28, 3)
+    assert 100.0 < gbs < 20000.0
+    with pytest.raises(BDError):
+        stream_bandwidth(1024, 3)
