@@ -2359,6 +2359,9 @@ __global__ void __launch_bounds__(kGderThreads)
     si = i < R.nb ? 0 : 1;
   }
   const double side = R.side, side_inv = R.side_inv;
+  // sqrt(d2) < range implies d2 < range^2 <= guard: pairs at or beyond the
+  // guard are never counted, and their square root is not taken
+  const double guard = range * range * (1.0 + 1e-12);
   unsigned int* wh = hist + (size_t)wave * 4 * nbins;
   const int j0 = (int)blockIdx.y * j_chunk;
   const int j1 = min(R.n, j0 + j_chunk);
@@ -2382,7 +2385,9 @@ __global__ void __launch_bounds__(kGderThreads)
         dx -= side * floor(side_inv * dx + 0.5);
         dy -= side * floor(side_inv * dy + 0.5);
         dz -= side * floor(side_inv * dz + 0.5);
-        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 < guard)) continue;
+        const double dist = sqrt(d2);
         if (dist < range) {
           int bin = (int)(dist / bin_size);
           if (bin == nbins) bin--;
@@ -2400,7 +2405,6 @@ __global__ void __launch_bounds__(kGderThreads)
   }
 }
 
-// Stand-alone rannew64 lanes for the stream test.
 // Stream copy for the bandwidth probe: b = a over n double4, grid-stride.
 __global__ void __launch_bounds__(256)
     k_stream_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
@@ -2408,6 +2412,7 @@ __global__ void __launch_bounds__(256)
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) b[k] = a[k];
 }
 
+// Stand-alone rannew64 lanes for the stream test.
 __global__ void k_rng_stream(uint64_t seed, int lanes, int ndraws, uint64_t* __restrict__ ring,
                              uint64_t* __restrict__ out) {
   const int l = (int)(blockIdx.x * blockDim.x + threadIdx.x);
