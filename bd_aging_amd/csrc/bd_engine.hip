@@ -432,8 +432,17 @@ __device__ __forceinline__ void k_cells_block(
     const int z = z0 + zl;
     const bool live = row_live && z < n;
     const int c = row * n + z;  // z + n (x + n y): coalesced across the lanes
+    const bool fast8 = ntags == 8 && (R.tag_off[g] & 3) == 0;
+    const int* slots = ins_g + (size_t)c * ntags;
     int occ = 0;
-    if (live) occ = nocup[R.cell_off[g] + c];
+    int4 slo = make_int4(0, 0, 0, 0), shi = make_int4(0, 0, 0, 0);
+    if (live) {
+      occ = nocup[R.cell_off[g] + c];
+      if (fast8) {  // the cell's eight slots, loaded beside its occupancy
+        slo = reinterpret_cast<const int4*>(slots)[0];
+        shi = reinterpret_cast<const int4*>(slots)[1];
+      }
+    }
     const int m = occ < ntags ? occ : ntags;
     int incl = m;
 #pragma unroll
@@ -447,12 +456,9 @@ __device__ __forceinline__ void k_cells_block(
     const int start = row_base + carry + (incl - prior) - m;
     if (live) {
       rrow[z] = start;
-      const int* slots = ins_g + (size_t)c * ntags;
-      if (ntags == 8 && (R.tag_off[g] & 3) == 0) {
-        // the cell's eight slots in two aligned 16-byte loads, ranked in registers
-        const int4 lo = reinterpret_cast<const int4*>(slots)[0];
-        const int4 hi = reinterpret_cast<const int4*>(slots)[1];
-        int id[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+      if (fast8) {
+        // the cell's eight slots (two aligned 16-byte loads), ranked in registers
+        int id[8] = {slo.x, slo.y, slo.z, slo.w, shi.x, shi.y, shi.z, shi.w};
 #pragma unroll
         for (int k = 0; k < 8; k++) id[k] = k < m ? id[k] : 0x7fffffff;
 #pragma unroll
