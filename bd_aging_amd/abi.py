@@ -20,6 +20,8 @@ BD_E_NOMEM = 6
 BD_POT_WCA = 0
 BD_POT_WCA_MODIFIED = 1
 BD_POT_HARMONIC = 2
+BD_POT_WCA49_50 = 3
+BD_POT_WCA49_50_AO = 4
 BD_ORDER_REFERENCE = 1
 BD_ORDER_SPLIT4 = 4
 BD_ORDER_SPLIT8 = 8
@@ -43,6 +45,8 @@ class BDParams(C.Structure):
         ("dt", C.c_double), ("kappa", C.c_double),
         ("T0", C.c_double), ("Tf", C.c_double), ("t0", C.c_double), ("tf", C.c_double),
         ("period", C.c_double), ("seed", C.c_uint64), ("first_step", C.c_int64),
+        ("pot_amp", C.c_double), ("pot_inv_t", C.c_double),
+        ("sigma_pol", C.c_double), ("phi_pol", C.c_double),
     ]
 
 

@@ -32,8 +32,8 @@
 //     (main.cu:796-797 overwrote it);
 //   * an instant quench (t0 == tf) gives Tf instead of NaN (set_temp.cu:8-11).
 // The potential is the reference's active wca_modified (cutoff sigma,
-// potentials.cu:66-76) unless BD_POTENTIAL=wca selects standard WCA (cutoff
-// 2^(1/6) sigma).
+// potentials.cu:66-76) unless BD_POTENTIAL selects standard WCA (cutoff
+// 2^(1/6) sigma) or one of the 49-50 potentials (potentials.cu:9-52).
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -144,10 +144,37 @@ void write_init_config(const std::string& path, const InitConfig& c, const doubl
   std::fclose(f);
 }
 
-int potential_from_env() {
+double env_double(const char* name, double dflt) {
+  const char* v = std::getenv(name);
+  return v && *v ? std::strtod(v, nullptr) : dflt;
+}
+
+// BD_POTENTIAL: wca_modified (default), wca, wca49_50 or wca49_50_ao.  The
+// 49-50 constants come from BD_POT_AMP and BD_POT_INV_T (default: the
+// reference's commented AA = 50 (50/49)^50 and invT = 0.678575,
+// get_forces.cu:46-48), the depletion branch's from BD_SIGMA_POL and
+// BD_PHI_POL.
+void set_potential_from_env(bd_params& p) {
   const char* v = std::getenv("BD_POTENTIAL");
-  if (v && std::strcmp(v, "wca") == 0) return BD_POT_WCA;
-  return BD_POT_WCA_MODIFIED;
+  p.potential = BD_POT_WCA_MODIFIED;
+  if (!v || !*v || std::strcmp(v, "wca_modified") == 0) return;
+  if (std::strcmp(v, "wca") == 0) {
+    p.potential = BD_POT_WCA;
+  } else if (std::strcmp(v, "wca49_50") == 0) {
+    p.potential = BD_POT_WCA49_50;
+  } else if (std::strcmp(v, "wca49_50_ao") == 0) {
+    p.potential = BD_POT_WCA49_50_AO;
+  } else {
+    std::fprintf(stderr, "unknown BD_POTENTIAL '%s' (wca_modified, wca, wca49_50, wca49_50_ao)\n",
+                 v);
+    std::exit(2);
+  }
+  if (p.potential == BD_POT_WCA49_50 || p.potential == BD_POT_WCA49_50_AO) {
+    p.pot_amp = env_double("BD_POT_AMP", 50.0 * std::pow(50.0 / 49.0, 50.0));
+    p.pot_inv_t = env_double("BD_POT_INV_T", 0.678575);
+    p.sigma_pol = env_double("BD_SIGMA_POL", 0.0);
+    p.phi_pol = env_double("BD_PHI_POL", 0.0);
+  }
 }
 
 // ---------------------------------------------------------------- configs
@@ -249,7 +276,7 @@ int make_configs(int argc, char** argv) {
     std::vector<bd_params> ep(nconf);
     for (int c = 0; c < nconf; c++) {
       ep[c] = hp[c];
-      ep[c].potential = potential_from_env();
+      set_potential_from_env(ep[c]);
       ep[c].ntags_big = ep[c].ntags_sml = 12;
       ep[c].dt = dt;
       ep[c].kappa = 0.0;
@@ -389,7 +416,7 @@ int run_system() {
     std::memset(&p, 0, sizeof p);
     p.n_big = cfg[i].n_big;
     p.n_sml = cfg[i].n_sml;
-    p.potential = potential_from_env();
+    set_potential_from_env(p);
     p.ntags_big = tagb;
     p.ntags_sml = tags;
     p.temp_protocol = sine ? BD_TEMP_SINE : BD_TEMP_LINEAR;
@@ -568,7 +595,7 @@ int main(int argc, char** argv) {
     std::memset(&p, 0, sizeof p);
     p.n_big = cfg[i].n_big;
     p.n_sml = cfg[i].n_sml;
-    p.potential = potential_from_env();
+    set_potential_from_env(p);
     p.ntags_big = ntags_big;
     p.ntags_sml = ntags_sml;
     p.temp_protocol = sine ? BD_TEMP_SINE : BD_TEMP_LINEAR;

@@ -124,16 +124,55 @@ __device__ inline void gauss3(uint64_t d0, uint64_t d1, uint64_t d2, uint64_t d3
   z = r2 * c;
 }
 
-// Pair potentials (potentials.cu:54-76, FP64): e is the pair's energy term,
-// added to the running potential as pot = pot + e; nf is dU/dr.
-template <int KIND>
-__device__ inline void pair_energy(double kappa, double sigma, double dist_inv, double& e,
+// Pair potentials (potentials.cu, FP64): e is the pair's energy term, added
+// to the running potential as pot = pot + e; nf is dU/dr.  q holds the
+// per-system constants: kappa, amp_t = pot_amp * pot_inv_t, inv_t, sigma_pol,
+// phi_pol.
+//   WCA          potentials.cu:54-64
+//   WCA_MODIFIED potentials.cu:66-76
+//   WCA49_50     potentials.cu:9-23 (s^6 as (s^3)^2, as the host form does)
+//   WCA49_50_AO  potentials.cu:27-52 (s^6 as six factors; the depletion
+//                branch beyond r = sigma)
+// EPS = 1 and 1 * a == a, so the reference's EPS*AA*invT is amp_t and
+// -EPS*AA*invT is -amp_t bit for bit.
+__device__ inline void repulsion_49_50(double amp_t, double inv_t, double sor, double sor6,
+                                       double dist_inv, double& e, double& nf) {
+  double sor12 = sor6 * sor6;
+  double sor24 = sor12 * sor12;
+  double sor48 = sor24 * sor24;
+  double sor49 = sor48 * sor;
+  double sor50 = sor49 * sor;
+  e = amp_t * (sor50 - sor49) + inv_t;
+  nf = -amp_t * (50.0 * sor50 - 49.0 * sor48 * sor) * dist_inv;
+}
+
+template <int KIND, class P>
+__device__ inline void pair_energy(const P& q, double sigma, double dist_inv, double& e,
                                    double& nf) {
   if (KIND == BD_POT_HARMONIC) {
     double r = 1.0 / dist_inv;
     double delta = sigma - r;
-    e = (0.5 * kappa) * delta * delta;
-    nf = -kappa * delta;
+    e = (0.5 * q.kappa) * delta * delta;
+    nf = -q.kappa * delta;
+  } else if (KIND == BD_POT_WCA49_50) {
+    double sor = dist_inv * sigma;
+    double sor6 = sor * sor * sor;
+    sor6 = sor6 * sor6;
+    repulsion_49_50(q.amp_t, q.inv_t, sor, sor6, dist_inv, e, nf);
+  } else if (KIND == BD_POT_WCA49_50_AO) {
+    double size_ratio = sigma / q.sigma_pol;
+    double sor = dist_inv * sigma;
+    double ros = 1.0 / sor;
+    if (ros <= 1.0) {
+      double sor6 = sor * sor * sor * sor * sor * sor;
+      repulsion_49_50(q.amp_t, q.inv_t, sor, sor6, dist_inv, e, nf);
+    } else {
+      double ros3 = ros * ros * ros;
+      double eta_1 = size_ratio + 1.0;
+      double eta2 = eta_1 * eta_1;
+      e = -q.phi_pol * (eta2 * eta_1 - 1.5 * eta2 * ros + 0.5 * ros3);
+      nf = q.phi_pol * (1.5 * eta2 * ros - 1.5 * ros3) * dist_inv;
+    }
   } else {
     double sor = dist_inv * sigma;
     double sor6 = sor * sor * sor;
@@ -149,11 +188,11 @@ __device__ inline void pair_energy(double kappa, double sigma, double dist_inv, 
   }
 }
 
-template <int KIND>
-__device__ inline void potential(double kappa, double sigma, double dist_inv, double& pot,
+template <int KIND, class P>
+__device__ inline void potential(const P& q, double sigma, double dist_inv, double& pot,
                                  double& nf) {
   double e;
-  pair_energy<KIND>(kappa, sigma, dist_inv, e, nf);
+  pair_energy<KIND>(q, sigma, dist_inv, e, nf);
   pot = pot + e;
 }
 

@@ -87,6 +87,7 @@ struct RepDev {
   double rc2_same[2];
   double rc2_bs;
   double dt, kappa;
+  double amp_t, inv_t, sigma_pol, phi_pol;  // BD_POT_WCA49_50(_AO)
   double T0, Tf, t0, tf, period;
   int64_t first_step;
   uint64_t seed;
@@ -569,7 +570,7 @@ __device__ inline void scan_grid(const RepDev& R, int g, int nrange, double sigm
           if (dist2 < rc2) {
             const double dist_inv = sqrt(1.0 / dist2);
             double nf;
-            bd::potential<POT>(R.kappa, sigma, dist_inv, pot, nf);
+            bd::potential<POT>(R, sigma, dist_inv, pot, nf);
             const double qx = nf * dist_inv * dx;
             const double qy = nf * dist_inv * dy;
             const double qz = nf * dist_inv * dz;
@@ -710,7 +711,7 @@ __device__ inline void scan_fast(const RepDev& R, int n, double cs, double sigma
           if (dist2 < rc2) {
             const double dist_inv = sqrt(1.0 / dist2);
             double nf;
-            bd::potential<POT>(R.kappa, sigma, dist_inv, pot, nf);
+            bd::potential<POT>(R, sigma, dist_inv, pot, nf);
             const double qx = nf * dist_inv * dx;
             const double qy = nf * dist_inv * dy;
             const double qz = nf * dist_inv * dz;
@@ -747,7 +748,7 @@ __device__ inline void pair_add(const RepDev& R, double sigma, double dist2, dou
                                 double& pot) {
   const double dist_inv = sqrt(1.0 / dist2);
   double nf;
-  bd::potential<POT>(R.kappa, sigma, dist_inv, pot, nf);
+  bd::potential<POT>(R, sigma, dist_inv, pot, nf);
   const double qx = nf * dist_inv * dx;
   const double qy = nf * dist_inv * dy;
   const double qz = nf * dist_inv * dz;
@@ -1433,7 +1434,7 @@ __device__ inline void pair_terms_q(const RepDev& R, double x, double y, double 
   const double dist2 = dx * dx + dy * dy + dz * dz;
   const double dist_inv = sqrt(1.0 / dist2);
   double nf;
-  bd::pair_energy<POT>(R.kappa, (wc & kPassBit) ? d.sigma1 : d.sigma0, dist_inv, e, nf);
+  bd::pair_energy<POT>(R, (wc & kPassBit) ? d.sigma1 : d.sigma0, dist_inv, e, nf);
   qx = nf * dist_inv * dx;
   qy = nf * dist_inv * dy;
   qz = nf * dist_inv * dz;
@@ -2749,9 +2750,36 @@ void launch_slab_forces(bd_sys* s, const int4* tbl, int nblk) {
   }
 }
 
+// The 49-50 potentials run in three kernels: the single-wavefront slab walk
+// with deferral depth 16 (the row walk without deferral where slot indexes
+// outgrow the deferral records), and the row walk that records the
+// neighbours.  Every walk, split and brick gives the same bits, so the shape
+// settings need no further instances.
+template <int POT>
+void launch_forces_phs_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl,
+                         int nblk) {
+  const MoveArgs mv = move_args(s);
+  if (ncounts)
+    launch(s, k_forces<POT, true, 0, 0, false>, nblk, kBlk, s->reps_d, tbl, s->px, s->py,
+           s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, ncounts, nlist, maxn, mv);
+  else if (s->defer_ok)
+    launch(s, k_forces<POT, false, 16, 1, false>, nblk, kBlk, s->reps_d, tbl, s->px, s->py,
+           s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, ncounts, nlist, maxn, mv);
+  else
+    launch(s, k_forces<POT, false, 0, 0, false>, nblk, kBlk, s->reps_d, tbl, s->px, s->py,
+           s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, ncounts, nlist, maxn, mv);
+}
+
+bool phs_kind(int pot) { return pot == BD_POT_WCA49_50 || pot == BD_POT_WCA49_50_AO; }
+
 template <bool NEIGH>
 void launch_forces_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl, int nblk) {
-  if (NEIGH)
+  if (s->pot_kind == BD_POT_WCA49_50)
+    launch_forces_phs_t<BD_POT_WCA49_50>(s, NEIGH ? ncounts : nullptr, nlist, maxn, tbl, nblk);
+  else if (s->pot_kind == BD_POT_WCA49_50_AO)
+    launch_forces_phs_t<BD_POT_WCA49_50_AO>(s, NEIGH ? ncounts : nullptr, nlist, maxn, tbl,
+                                            nblk);
+  else if (NEIGH)
     launch_forces_k<true, 0, 0, false>(s, ncounts, nlist, maxn, tbl, nblk);
   else if (s->walk == BD_WALK_LIST)
     launch_list(s, tbl, nblk);
@@ -2814,7 +2842,7 @@ void launch_split_t(bd_sys* s) {
 // 3.02e9 on 256 replicas).
 bool fusable(const bd_sys* s) {
   if (s->fuse == 0 || s->order != BD_ORDER_REFERENCE || s->brick != 0 || s->walk < 1 ||
-      s->walk == BD_WALK_LIST)
+      s->walk == BD_WALK_LIST || phs_kind(s->pot_kind))
     return false;
   if (s->fuse == 2 && (s->pass_split == 0 || s->defer == 0)) return false;
   for (int r = 0; r < s->R; r++)
@@ -2828,7 +2856,9 @@ void launch_forces_move(bd_sys* s) {
 }
 
 void launch_forces(bd_sys* s) {
-  if (s->order == 4) {
+  if (phs_kind(s->pot_kind)) {
+    launch_forces_t<false>(s, nullptr, nullptr, 0, s->tbl_part, (int)s->tbl_part_h.size());
+  } else if (s->order == 4) {
     launch_split_t<4>(s);
   } else if (s->order == 8) {
     launch_split_t<8>(s);
@@ -2870,6 +2900,10 @@ int launch_persistent(bd_sys* s, int nsteps) {
       return launch_persistent_t<BD_POT_WCA>(s, nsteps);
     case BD_POT_WCA_MODIFIED:
       return launch_persistent_t<BD_POT_WCA_MODIFIED>(s, nsteps);
+    case BD_POT_WCA49_50:
+      return launch_persistent_t<BD_POT_WCA49_50>(s, nsteps);
+    case BD_POT_WCA49_50_AO:
+      return launch_persistent_t<BD_POT_WCA49_50_AO>(s, nsteps);
     default:
       return launch_persistent_t<BD_POT_HARMONIC>(s, nsteps);
   }
@@ -3110,6 +3144,22 @@ int bd_random_positions(int32_t n, double side, uint64_t seed, double* xyz) {
   return BD_OK;
 }
 
+// Cutoff of a pair with diameter sigma: 2^(1/6) sigma (WCA), (50/49) sigma
+// (49-50, get_forces.cu:47-49), sigma (1 + sigma / sigma_pol) where the
+// depletion term and its force vanish (49-50 + AO), else sigma.
+static double pair_cutoff(const bd_params* p, double sigma) {
+  switch (p->potential) {
+    case BD_POT_WCA:
+      return bd::kWcaCut * sigma;
+    case BD_POT_WCA49_50:
+      return (50.0 / 49.0) * sigma;
+    case BD_POT_WCA49_50_AO:
+      return sigma * (1.0 + sigma / p->sigma_pol);
+    default:
+      return 1.0 * sigma;
+  }
+}
+
 // main.cu:259-298 with the cutoff rc of the chosen potential in place of
 // sigma: ncell = (int)(L / (1.05 rc)), cell side L/ncell, and
 // nrange = 1 + (int)(rc / cell side of the scanned grid).
@@ -3120,13 +3170,20 @@ int bd_compute_geometry(const bd_params* p, bd_geometry* g) {
   if (p->n_big < 1 || p->n_sml < 1) return fail(BD_E_ARG, "both species need particles");
   if (p->ntags_big < 1 || p->ntags_big > 32 || p->ntags_sml < 1 || p->ntags_sml > 32)
     return fail(BD_E_ARG, "ntags must be in 1..32");
-  if (p->potential < BD_POT_WCA || p->potential > BD_POT_HARMONIC)
+  if (p->potential < BD_POT_WCA || p->potential > BD_POT_WCA49_50_AO)
     return fail(BD_E_ARG, "unknown potential");
-  const double c = p->potential == BD_POT_WCA ? bd::kWcaCut : 1.0;
+  if (phs_kind(p->potential) &&
+      !(p->pot_amp > 0.0 && p->pot_inv_t > 0.0 && std::isfinite(p->pot_amp * p->pot_inv_t)))
+    return fail(BD_E_ARG, "the 49-50 potentials need finite pot_amp > 0 and pot_inv_t > 0");
+  if (p->potential == BD_POT_WCA49_50_AO &&
+      !(p->sigma_pol > 0.0 && p->phi_pol >= 0.0 && std::isfinite(p->phi_pol)))
+    return fail(BD_E_ARG, "the depletion branch needs sigma_pol > 0 and finite phi_pol >= 0");
   const double sbs = 0.5 * (p->sigma_big + p->sigma_sml);
-  g->rc_bb = c * p->sigma_big;
-  g->rc_ss = c * p->sigma_sml;
-  g->rc_bs = c * sbs;
+  g->rc_bb = pair_cutoff(p, p->sigma_big);
+  g->rc_ss = pair_cutoff(p, p->sigma_sml);
+  g->rc_bs = pair_cutoff(p, sbs);
+  if (!std::isfinite(g->rc_bb) || !std::isfinite(g->rc_ss) || !std::isfinite(g->rc_bs))
+    return fail(BD_E_ARG, "cutoff not finite");
   g->ncell_big = (int32_t)(p->side / (1.05 * g->rc_bb));
   g->ncell_sml = (int32_t)(p->side / (1.05 * g->rc_ss));
   if (g->ncell_big < 1 || g->ncell_sml < 1) return fail(BD_E_ARG, "box smaller than a cell");
@@ -3217,6 +3274,10 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
     R.rc2_bs = g.rc_bs * g.rc_bs;
     R.dt = p.dt;
     R.kappa = p.kappa;
+    R.amp_t = p.pot_amp * p.pot_inv_t;
+    R.inv_t = p.pot_inv_t;
+    R.sigma_pol = p.sigma_pol;
+    R.phi_pol = p.phi_pol;
     R.T0 = p.T0;
     R.Tf = p.Tf;
     R.t0 = p.t0;
@@ -3668,6 +3729,8 @@ int bd_set_force_order(bd_sys* s, int32_t order) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (order != BD_ORDER_REFERENCE && order != BD_ORDER_SPLIT4 && order != BD_ORDER_SPLIT8)
     return fail(BD_E_ARG, "order must be BD_ORDER_REFERENCE, _SPLIT4 or _SPLIT8");
+  if (order != BD_ORDER_REFERENCE && phs_kind(s->pot_kind))
+    return fail(BD_E_ARG, "the 49-50 potentials run in the reference order only");
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (s->graph) {
     (void)hipGraphExecDestroy(s->graph);
@@ -3777,6 +3840,10 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
                                            c.walk != keep.walk || c.split != keep.split))
       continue;
     if (c.defer != 0 && !s->defer_ok) continue;
+    // the 49-50 potentials run one kernel whatever the shape: time it once
+    if (phs_kind(s->pot_kind) && (c.brick != 0 || c.defer != 16 || c.walk != BD_WALK_SLABS ||
+                                  c.split != 0))
+      continue;
     if (setup_bricks(s, c.brick) != BD_OK) continue;
     s->walk = c.walk;
     s->defer = c.defer;

@@ -22,10 +22,17 @@ def box_side(n_big, n_sml, sigma_big, sigma_sml, phi):
     return (vol / phi) ** (1.0 / 3.0)
 
 
+# The reference's commented-out 49-50 parameters (get_forces.cu:46-48):
+# AA = 50 Apow(50/49) with Apow(x) = x^50 (get_forces.cu:7-17), invT = 0.678575.
+PHS_AMP = 50.0 * (50.0 / 49.0) ** 50
+PHS_INV_T = 0.678575
+
+
 def make_params(n_big, n_sml, side=None, phi=None, sigma_big=1.4, sigma_sml=1.0, dt=1e-4,
                 potential=abi.BD_POT_WCA, ntags_big=8, ntags_sml=8,
                 temp_protocol=abi.BD_TEMP_LINEAR, T0=1.0, Tf=1.0, t0=0.0, tf=0.0, period=1.0,
-                kappa=0.0, seed=12345, first_step=0, rng_lanes=0):
+                kappa=0.0, seed=12345, first_step=0, rng_lanes=0, pot_amp=None,
+                pot_inv_t=None, sigma_pol=0.0, phi_pol=0.0):
     if side is None:
         if phi is None:
             raise ValueError("give side or phi")
@@ -42,6 +49,10 @@ def make_params(n_big, n_sml, side=None, phi=None, sigma_big=1.4, sigma_sml=1.0,
     p.T0, p.Tf, p.t0, p.tf, p.period = float(T0), float(Tf), float(t0), float(tf), float(period)
     p.seed = int(seed) & ((1 << 64) - 1)
     p.first_step = int(first_step)
+    phs = potential in (abi.BD_POT_WCA49_50, abi.BD_POT_WCA49_50_AO)
+    p.pot_amp = float(PHS_AMP if pot_amp is None and phs else (pot_amp or 0.0))
+    p.pot_inv_t = float(PHS_INV_T if pot_inv_t is None and phs else (pot_inv_t or 0.0))
+    p.sigma_pol, p.phi_pol = float(sigma_pol), float(phi_pol)
     return p
 
 

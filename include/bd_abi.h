@@ -49,8 +49,20 @@ enum {
 };
 
 /* pair potentials (potentials.cu:54-76); HARMONIC is the overlap-removal
- * potential of the initial-configuration generator */
-enum { BD_POT_WCA = 0, BD_POT_WCA_MODIFIED = 1, BD_POT_HARMONIC = 2 };
+ * potential of the initial-configuration generator.  WCA49_50 is the
+ * pseudo-hard-sphere potential_wca49_50_hst (potentials.cu:9-23), cutoff
+ * (50/49) sigma_ij; WCA49_50_AO adds the Asakura-Oosawa depletion branch of
+ * potential_wca49_50_AO_hst (potentials.cu:27-52) beyond r = sigma_ij, cutoff
+ * sigma_ij (1 + sigma_ij / sigma_pol), where that branch's energy and force
+ * reach zero.  The reference only holds them as commented-out call sites
+ * (get_forces.cu:44-49,119). */
+enum {
+  BD_POT_WCA = 0,
+  BD_POT_WCA_MODIFIED = 1,
+  BD_POT_HARMONIC = 2,
+  BD_POT_WCA49_50 = 3,
+  BD_POT_WCA49_50_AO = 4
+};
 /* temperature protocols (set_temp.cu:7-23) */
 enum { BD_TEMP_LINEAR = 0, BD_TEMP_SINE = 1 };
 enum { BD_SPECIES_BIG = 0, BD_SPECIES_SML = 1 };
@@ -76,6 +88,17 @@ typedef struct bd_params {
   double period;          /* sine protocol only */
   uint64_t seed;          /* master seed of the rannew64 lanes */
   int64_t first_step;     /* step index ni of the first update (-ntrans) */
+  /* BD_POT_WCA49_50(_AO) only: U = pot_inv_t (pot_amp (s^50 - s^49) + 1),
+   * s = sigma/r, for r below the cutoff (below sigma with _AO).  The
+   * reference's commented values are pot_amp = 50 (50/49)^50 and
+   * pot_inv_t = 0.678575 (get_forces.cu:46-48). */
+  double pot_amp;
+  double pot_inv_t;
+  /* BD_POT_WCA49_50_AO only: polymer diameter and volume fraction of
+   * U = -phi_pol (e^3 - 1.5 e^2 x + 0.5 x^3), e = 1 + sigma/sigma_pol,
+   * x = r/sigma, for sigma < r < e sigma (potentials.cu:43-50) */
+  double sigma_pol;
+  double phi_pol;
 } bd_params;
 
 /* Cell geometry derived from bd_params (main.cu:259-298 with the cutoff of

@@ -199,7 +199,16 @@ void orc_gauss3(const uint64_t d[4], double out[3]) {
  * sigma_sml).  A grid narrower than 2*nrange+1 cells would visit a cell
  * twice, so it is rejected.
  * ==================================================================== */
-static double cut_factor(int32_t kind) { return kind == BD_POT_WCA ? WCA_CUT : 1.0; }
+/* Pair cutoff: 2^(1/6) sigma (WCA); (50/49) sigma (49-50, the commented
+ * cutoff=BB*sigma of get_forces.cu:47-49); sigma (1 + sigma/sigma_pol) for
+ * 49-50 + AO, the distance at which the depletion term of
+ * potentials.cu:43-50 and its force reach zero; sigma otherwise. */
+static double pair_cut(const bd_params* p, double sigma) {
+  if (p->potential == BD_POT_WCA) return WCA_CUT * sigma;
+  if (p->potential == BD_POT_WCA49_50) return (50.0 / 49.0) * sigma;
+  if (p->potential == BD_POT_WCA49_50_AO) return sigma * (1.0 + sigma / p->sigma_pol);
+  return 1.0 * sigma;
+}
 
 int orc_geometry(const bd_params* p, bd_geometry* g) {
   if (!p || !g) return BD_E_ARG;
@@ -207,12 +216,18 @@ int orc_geometry(const bd_params* p, bd_geometry* g) {
   if (p->n_big < 1 || p->n_sml < 1) return BD_E_ARG;
   if (p->ntags_big < 1 || p->ntags_big > 32 || p->ntags_sml < 1 || p->ntags_sml > 32)
     return BD_E_ARG;
-  if (p->potential < BD_POT_WCA || p->potential > BD_POT_HARMONIC) return BD_E_ARG;
-  double c = cut_factor(p->potential);
+  if (p->potential < BD_POT_WCA || p->potential > BD_POT_WCA49_50_AO) return BD_E_ARG;
+  if ((p->potential == BD_POT_WCA49_50 || p->potential == BD_POT_WCA49_50_AO) &&
+      !(p->pot_amp > 0.0 && p->pot_inv_t > 0.0 && isfinite(p->pot_amp * p->pot_inv_t)))
+    return BD_E_ARG;
+  if (p->potential == BD_POT_WCA49_50_AO &&
+      !(p->sigma_pol > 0.0 && p->phi_pol >= 0.0 && isfinite(p->phi_pol)))
+    return BD_E_ARG;
   double sbs = 0.5 * (p->sigma_big + p->sigma_sml);
-  g->rc_bb = c * p->sigma_big;
-  g->rc_ss = c * p->sigma_sml;
-  g->rc_bs = c * sbs;
+  g->rc_bb = pair_cut(p, p->sigma_big);
+  g->rc_ss = pair_cut(p, p->sigma_sml);
+  g->rc_bs = pair_cut(p, sbs);
+  if (!isfinite(g->rc_bb) || !isfinite(g->rc_ss) || !isfinite(g->rc_bs)) return BD_E_ARG;
   g->ncell_big = (int32_t)(p->side / (1.05 * g->rc_bb));
   g->ncell_sml = (int32_t)(p->side / (1.05 * g->rc_ss));
   if (g->ncell_big < 1 || g->ncell_sml < 1) return BD_E_ARG;
@@ -297,20 +312,60 @@ int32_t orc_sampling_indexes(float minimal_decade, float run_time, float dt,
 }
 
 /* ======================================================================
- * Pair potentials (potentials.cu:54-76) in FP64.  normal_force is dU/dr.
+ * Pair potentials (potentials.cu) in FP64.  normal_force is dU/dr; the
+ * energy term is added as potential = potential + term.
  * HARMONIC (overlap removal, no reference counterpart):
  *   U = kappa/2 (sigma - r)^2 for r < sigma.
+ * WCA49_50 follows potential_wca49_50_hst (potentials.cu:9-23) and
+ * WCA49_50_AO potential_wca49_50_AO_hst (potentials.cu:27-52) with
+ * AA*invT = pot_amp*pot_inv_t; EPS = 1, so EPS*AA*invT is that product and
+ * -EPS*AA*invT its negation, bit for bit.  The two host forms build s^6
+ * differently ((s^3)^2 and six factors); each is kept.
  * ==================================================================== */
-void orc_potential(int32_t kind, double kappa, double sigma, double dist_inv,
-                   double* potential, double* normal_force) {
+static void repulsion_49_50(double ai, double inv_t, double sor, double sor6, double dist_inv,
+                            double* potential, double* normal_force) {
+  double sor12 = sor6 * sor6;
+  double sor24 = sor12 * sor12;
+  double sor48 = sor24 * sor24;
+  double sor50 = sor48 * sor * sor;
+  *potential = *potential + (ai * (sor50 - sor48 * sor) + inv_t);
+  *normal_force = -ai * (50.0 * sor50 - 49.0 * sor48 * sor) * dist_inv;
+}
+
+void orc_potential(const bd_params* p, double sigma, double dist_inv, double* potential,
+                   double* normal_force) {
+  int32_t kind = p->potential;
   if (kind == BD_POT_HARMONIC) {
     double r = 1.0 / dist_inv;
     double delta = sigma - r;
-    *potential = *potential + (0.5 * kappa) * delta * delta;
-    *normal_force = -kappa * delta;
+    *potential = *potential + (0.5 * p->kappa) * delta * delta;
+    *normal_force = -p->kappa * delta;
     return;
   }
   double sor = dist_inv * sigma;
+  if (kind == BD_POT_WCA49_50) {
+    double sor6 = sor * sor * sor;
+    sor6 = sor6 * sor6;
+    repulsion_49_50(p->pot_amp * p->pot_inv_t, p->pot_inv_t, sor, sor6, dist_inv, potential,
+                    normal_force);
+    return;
+  }
+  if (kind == BD_POT_WCA49_50_AO) {
+    double size_ratio = sigma / p->sigma_pol;
+    double ros = 1.0 / sor;
+    if (ros <= 1.0) {
+      double sor6 = sor * sor * sor * sor * sor * sor;
+      repulsion_49_50(p->pot_amp * p->pot_inv_t, p->pot_inv_t, sor, sor6, dist_inv, potential,
+                      normal_force);
+    } else {
+      double ros3 = ros * ros * ros;
+      double eta_1 = size_ratio + 1.0;
+      double eta2 = eta_1 * eta_1;
+      *potential = *potential + (-p->phi_pol * (eta2 * eta_1 - 1.5 * eta2 * ros + 0.5 * ros3));
+      *normal_force = p->phi_pol * (1.5 * eta2 * ros - 1.5 * ros3) * dist_inv;
+    }
+    return;
+  }
   double sor6 = sor * sor * sor;
   sor6 = sor6 * sor6;
   double sor12 = sor6 * sor6;
@@ -534,7 +589,7 @@ static void scan_grid(const orc_sys* s, int32_t m, int32_t grid, int32_t nrange,
           if (dist2 < rc2) {
             double dist_inv = sqrt(1.0 / dist2);
             double nf;
-            orc_potential(s->p.potential, s->p.kappa, sigma, dist_inv, pot, &nf);
+            orc_potential(&s->p, sigma, dist_inv, pot, &nf);
             double px = nf * dist_inv * dx;
             double py = nf * dist_inv * dy;
             double pz = nf * dist_inv * dz;
@@ -616,7 +671,7 @@ int orc_forces_brute(const bd_params* p, const double* xyz, double* f, double* v
       double d2 = dx * dx + dy * dy + dz * dz;
       if (d2 < rc * rc) {
         double di = sqrt(1.0 / d2), nf;
-        orc_potential(p->potential, p->kappa, sigma, di, &u, &nf);
+        orc_potential(p, sigma, di, &u, &nf);
         double px = nf * di * dx, py = nf * di * dy, pz = nf * di * dz;
         fx += px;
         fy += py;

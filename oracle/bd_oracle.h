@@ -42,9 +42,9 @@ double orc_temperature(const bd_params* p, int64_t ni);
 int32_t orc_sampling_indexes(float minimal_decade, float run_time, float dt,
                              int32_t per_decade, int32_t* out, int32_t maxn);
 
-/* ---- pair potentials (potentials.cu:54-76) ---- */
-void orc_potential(int32_t kind, double kappa, double sigma, double dist_inv,
-                   double* potential, double* normal_force);
+/* ---- pair potentials (potentials.cu:9-76); kind and constants from p ---- */
+void orc_potential(const bd_params* p, double sigma, double dist_inv, double* potential,
+                   double* normal_force);
 
 /* ---- one system ---- */
 typedef struct orc_sys orc_sys;

@@ -39,7 +39,7 @@ _SIG = {
     "orc_temperature": (C.c_double, [C.POINTER(BDParams), C.c_int64]),
     "orc_sampling_indexes": (C.c_int32, [C.c_float, C.c_float, C.c_float, C.c_int32, _I,
                                          C.c_int32]),
-    "orc_potential": (None, [C.c_int32, C.c_double, C.c_double, C.c_double, _D, _D]),
+    "orc_potential": (None, [C.POINTER(BDParams), C.c_double, C.c_double, _D, _D]),
     "orc_create": (_P, [C.POINTER(BDParams)]),
     "orc_destroy": (None, [_P]),
     "orc_get_geometry": (C.c_int, [_P, C.POINTER(BDGeometry)]),

@@ -36,8 +36,8 @@ def test_python_signatures_cover_header():
 
 def test_geometry_matches_oracle():
     for n, phi, pot in [(2000, 0.5, 0), (32768, 0.58, 0), (32768, 0.58, 1), (500, 0.2, 2),
-                        (262144, 0.58, 0)]:
-        p = make_params(n // 2, n - n // 2, phi=phi, potential=pot)
+                        (262144, 0.58, 0), (2000, 0.5, 3), (2000, 0.5, 4), (32768, 0.58, 4)]:
+        p = make_params(n // 2, n - n // 2, phi=phi, potential=pot, sigma_pol=4.0, phi_pol=0.3)
         g = geometry(p)
         rc, go = orc.geometry(p)
         assert rc == 0
@@ -50,6 +50,12 @@ def test_geometry_rejects_bad_params():
     with pytest.raises(abi.BDError):
         geometry(p)
     p = make_params(100, 100, phi=0.3, ntags_big=40)
+    with pytest.raises(abi.BDError):
+        geometry(p)
+    p = make_params(1000, 1000, phi=0.5, potential=abi.BD_POT_WCA49_50_AO)  # no sigma_pol
+    with pytest.raises(abi.BDError):
+        geometry(p)
+    p = make_params(1000, 1000, phi=0.5, potential=5)
     with pytest.raises(abi.BDError):
         geometry(p)
 

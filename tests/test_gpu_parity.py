@@ -13,7 +13,8 @@ import pytest
 
 import orc
 from bd_aging_amd import abi
-from bd_aging_amd.abi import BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC, BD_TEMP_SINE, BDError
+from bd_aging_amd.abi import (BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC, BD_POT_WCA49_50,
+                              BD_POT_WCA49_50_AO, BD_TEMP_SINE, BDError)
 from bd_aging_amd.engine import Engine, make_params, rng_stream, box_side
 from bd_aging_amd.init import relax_overlaps
 
@@ -86,13 +87,25 @@ def test_cells_edge_positions():
 
 
 # ------------------------------------------------------------------ forces
-@pytest.mark.parametrize("pot", [BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC])
-def test_forces_and_neighbours(config1_positions, pot):
+# sigma_pol = 4: cutoffs (1.89, 1.56, 1.25) keep the stencils within two
+# cells (the slab walk); sigma_pol = 1: cutoffs (3.36, 2.64, 2.0) need the
+# general stencil path, and its 4^3 big cells hold about 16 particles each.
+AO_FAST = dict(sigma_pol=4.0, phi_pol=0.3)
+AO_WIDE = dict(sigma_pol=1.0, phi_pol=0.4, ntags_big=32, ntags_sml=16)
+
+
+@pytest.mark.parametrize("pot,kw", [(BD_POT_WCA, {}), (BD_POT_WCA_MODIFIED, {}),
+                                    (BD_POT_HARMONIC, {}), (BD_POT_WCA49_50, {}),
+                                    (BD_POT_WCA49_50_AO, AO_FAST),
+                                    (BD_POT_WCA49_50_AO, AO_WIDE)])
+def test_forces_and_neighbours(config1_positions, pot, kw):
     side, x = config1_positions
-    p = make_params(1000, 1000, side=side, potential=pot, kappa=30.0)
+    p = make_params(1000, 1000, side=side, potential=pot, kappa=30.0, **kw)
     xx = x.copy()
-    if pot == BD_POT_HARMONIC:  # make some overlaps so the harmonic force is non-zero
-        xx[:50] += 0.05
+    if pot in (BD_POT_HARMONIC, BD_POT_WCA49_50, BD_POT_WCA49_50_AO):
+        # move some particles into contact so the short-range forces act
+        # (the relaxed configuration keeps pairs beyond 2^(1/6) sigma)
+        xx[:50] += 0.05 if pot == BD_POT_HARMONIC else 0.15
         xx %= side
     e, o = pair(p, xx)
     e.cell_locate()
@@ -107,8 +120,8 @@ def test_forces_and_neighbours(config1_positions, pot):
     assert_forces_close(ue, uo)
     exact = np.array_equal(fe, fo) and np.array_equal(ve, vo) and np.array_equal(ue, uo)
     print(f"forces bit-exact: {exact}")
-    ce, le = e.neighbours(64)
-    co, lo = o.neighbours(64)
+    ce, le = e.neighbours(256)
+    co, lo = o.neighbours(256)
     assert np.array_equal(ce, co)
     for m in range(len(x)):
         assert np.array_equal(le[m, :ce[m]], lo[m, :co[m]])
@@ -362,6 +375,35 @@ def test_trajectory_100_steps(config1_positions, lanes):
     fo, _, _ = o.forces()
     assert np.allclose(fe, fo, rtol=1e-9, atol=1e-9 * np.abs(fo).max())
     assert e.step_index() == o.step_index() == 100
+    e.close()
+
+
+@pytest.mark.parametrize("pot,kw", [(BD_POT_WCA49_50, {}), (BD_POT_WCA49_50_AO, AO_FAST),
+                                    (BD_POT_WCA49_50_AO, AO_WIDE)])
+def test_49_50_trajectory_100_steps(config1_positions, pot, kw):
+    """potentials.cu:9-52 in the step: 100 steps against the oracle, whatever
+    the shape settings (they select the same kernels for these potentials).
+    dt = 1e-5: the s^50 wall is too stiff for explicit steps of 1e-4 (a 5 %
+    overlap then moves a particle by 0.2 sigma in one step)."""
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, T0=1.0, Tf=1.0, seed=17, potential=pot, dt=1e-5, **kw)
+    e, o = pair(p, x)
+    e.set_walk(abi.BD_WALK_CHUNK4)
+    e.set_pass_split(2)
+    e.set_fused_update(1)
+    e.step(100)
+    assert o.cell_locate() == 0
+    o.compute_forces()
+    assert o.step(100) == 0
+    pe, po = e.positions(), o.positions()
+    assert np.array_equal(pe, po)
+    assert np.array_equal(e.raw_positions(), o.raw_positions())
+    fe, ve, ue = e.forces()
+    fo, vo, uo = o.forces()
+    assert np.abs(fo).max() > 0
+    assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and np.array_equal(ue, uo)
+    with pytest.raises(BDError):
+        e.set_force_order(abi.BD_ORDER_SPLIT4)
     e.close()
 
 

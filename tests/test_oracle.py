@@ -13,7 +13,8 @@ import pytest
 
 import orc
 from bd_aging_amd.engine import make_params, box_side
-from bd_aging_amd.abi import BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC, BD_TEMP_SINE
+from bd_aging_amd.abi import (BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC, BD_POT_WCA49_50,
+                              BD_POT_WCA49_50_AO, BD_TEMP_SINE)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -181,9 +182,14 @@ def test_cell_overflow_reported():
 
 
 # ---------------------------------------------------------------- forces
-@pytest.mark.parametrize("pot", [BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC])
+AO_KW = dict(sigma_pol=4.0, phi_pol=0.3)
+
+
+@pytest.mark.parametrize("pot", [BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC,
+                                 BD_POT_WCA49_50, BD_POT_WCA49_50_AO])
 def test_forces_match_brute_force(pot):
-    p, x = random_dilute(90, 110, 0.25, 11, pot=pot, kappa=50.0)
+    kw = AO_KW if pot == BD_POT_WCA49_50_AO else {}
+    p, x = random_dilute(90, 110, 0.25, 11, pot=pot, kappa=50.0, **kw)
     s = orc.System(p)
     s.set_positions(x)
     s.cell_locate()
@@ -212,11 +218,12 @@ def test_forces_match_brute_force(pot):
         assert set(lst[m, :counts[m]].tolist()) == exp
 
 
-def pot_eval(kind, sigma, r, kappa=0.0):
+def pot_eval(kind, sigma, r, kappa=0.0, **kw):
     import ctypes as C
+    p = make_params(4, 4, side=10.0, potential=kind, kappa=kappa, **kw)
     u = C.c_double(0.0)
     nf = C.c_double(0.0)
-    orc.lib().orc_potential(kind, kappa, sigma, 1.0 / r, C.byref(u), C.byref(nf))
+    orc.lib().orc_potential(C.byref(p), sigma, 1.0 / r, C.byref(u), C.byref(nf))
     return u.value, nf.value
 
 
@@ -237,6 +244,86 @@ def test_potential_closed_forms():
         assert math.isclose(u, 1.0) and math.isclose(nf, -24.0 / sigma)
         u, nf = pot_eval(BD_POT_HARMONIC, sigma, 0.9 * sigma, kappa=10.0)
         assert math.isclose(u, 5.0 * (0.1 * sigma) ** 2) and math.isclose(nf, -10.0 * 0.1 * sigma)
+
+
+def fd_slope(kind, sigma, r, h, **kw):
+    """Central difference of the energy: dU/dr to O(h^2)."""
+    return (pot_eval(kind, sigma, r + h, **kw)[0] - pot_eval(kind, sigma, r - h, **kw)[0]) / (2 * h)
+
+
+@pytest.mark.parametrize("kind", [BD_POT_WCA49_50, BD_POT_WCA49_50_AO])
+def test_49_50_repulsion_pinned(kind):
+    """potentials.cu:9-23 / :27-41: U = invT (A (s^50 - s^49) + 1), s = sigma/r."""
+    from bd_aging_amd.engine import PHS_AMP, PHS_INV_T
+    kw = AO_KW if kind == BD_POT_WCA49_50_AO else {}
+    for sigma in (1.0, 1.2, 1.4):
+        # contact: s = 1, the bracket vanishes, dU/dr = -A invT / sigma
+        u, nf = pot_eval(kind, sigma, sigma, **kw)
+        assert u == PHS_INV_T
+        assert math.isclose(nf, -PHS_AMP * PHS_INV_T / sigma, rel_tol=1e-14)
+        # the force is minus the slope of the energy (exponents, signs)
+        top = sigma * (50.0 / 49.0 if kind == BD_POT_WCA49_50 else 1.0)
+        for r in np.linspace(0.97 * sigma, top - 1e-3 * sigma, 6):
+            h = 1e-6 * sigma
+            assert math.isclose(pot_eval(kind, sigma, r, **kw)[1], fd_slope(kind, sigma, r, h, **kw),
+                                rel_tol=1e-6, abs_tol=1e-6)
+        # closed form at a few s, exact rationals
+        for r in (0.98 * sigma, 0.995 * sigma):
+            sr = Fraction(sigma) / Fraction(r)
+            exp_u = PHS_INV_T * (PHS_AMP * float(sr ** 50 - sr ** 49) + 1.0)
+            assert math.isclose(pot_eval(kind, sigma, r, **kw)[0], exp_u, rel_tol=1e-12)
+    if kind == BD_POT_WCA49_50:
+        # the force vanishes at the cutoff (50/49) sigma: 50 s^50 - 49 s^49 = 0
+        u, nf = pot_eval(kind, 1.0, 50.0 / 49.0)
+        assert abs(nf) < 1e-12 * PHS_AMP
+        # with the reference's A = 50 (50/49)^50 the energy there is -invT/49;
+        # with A = 50 (50/49)^49 it is shifted to zero
+        assert math.isclose(u, -PHS_INV_T / 49.0, rel_tol=1e-12)
+        u0, _ = pot_eval(kind, 1.0, 50.0 / 49.0, pot_amp=50.0 * (50.0 / 49.0) ** 49)
+        assert abs(u0) < 1e-12
+
+
+def test_ao_depletion_branch_pinned():
+    """potentials.cu:43-50: beyond contact U = -phi_pol (e^3 - 1.5 e^2 x + 0.5 x^3),
+    e = 1 + sigma/sigma_pol, x = r/sigma."""
+    for sigma, spol, phi in ((1.0, 1.0, 0.4), (1.4, 4.0, 0.3), (1.2, 2.5, 0.1)):
+        kw = dict(sigma_pol=spol, phi_pol=phi)
+        eta = 1.0 + sigma / spol
+        # energy and force reach zero at the cutoff e sigma
+        u, nf = pot_eval(BD_POT_WCA49_50_AO, sigma, eta * sigma, **kw)
+        assert abs(u) < 1e-12 and abs(nf) < 1e-12
+        for r in np.linspace(1.001 * sigma, 0.999 * eta * sigma, 7):
+            u, nf = pot_eval(BD_POT_WCA49_50_AO, sigma, r, **kw)
+            assert u < 0  # attraction
+            h = 1e-6 * sigma
+            assert math.isclose(nf, fd_slope(BD_POT_WCA49_50_AO, sigma, r, h, **kw),
+                                rel_tol=1e-6, abs_tol=1e-9)
+            if spol == sigma:
+                # equal sizes: the textbook Asakura-Oosawa overlap volume,
+                # U = -phi (1+q)^3/q^3 [1 - 3x/(2(1+q)) + x^3/(2(1+q)^3)], q = 1
+                x = r / sigma
+                exp = -phi * 8.0 * (1.0 - 3.0 * x / 4.0 + x ** 3 / 16.0)
+                assert math.isclose(u, exp, rel_tol=1e-12, abs_tol=1e-14)
+        # inside contact the 49-50 repulsion applies (same value as kind 3)
+        u, nf = pot_eval(BD_POT_WCA49_50_AO, sigma, 0.99 * sigma, **kw)
+        u3, nf3 = pot_eval(BD_POT_WCA49_50, sigma, 0.99 * sigma)
+        assert math.isclose(u, u3, rel_tol=1e-14) and math.isclose(nf, nf3, rel_tol=1e-14)
+
+
+def test_49_50_geometry():
+    p = make_params(1000, 1000, phi=0.5, potential=BD_POT_WCA49_50)
+    rc, g = orc.geometry(p)
+    assert rc == 0
+    assert (g.rc_bb, g.rc_ss, g.rc_bs) == tuple((50.0 / 49.0) * s for s in (1.4, 1.0, 1.2))
+    p = make_params(1000, 1000, phi=0.5, potential=BD_POT_WCA49_50_AO, sigma_pol=4.0, phi_pol=0.3)
+    rc, g = orc.geometry(p)
+    assert rc == 0
+    assert (g.rc_bb, g.rc_ss, g.rc_bs) == tuple(s * (1.0 + s / 4.0) for s in (1.4, 1.0, 1.2))
+    for bad in (dict(sigma_pol=0.0, phi_pol=0.3), dict(sigma_pol=4.0, phi_pol=-1.0)):
+        p = make_params(1000, 1000, phi=0.5, potential=BD_POT_WCA49_50_AO, **bad)
+        assert orc.geometry(p)[0] != 0
+    p = make_params(1000, 1000, phi=0.5, potential=BD_POT_WCA49_50, pot_inv_t=-1.0)
+    assert orc.geometry(p)[0] != 0
 
 
 # ---------------------------------------------------------------- update
