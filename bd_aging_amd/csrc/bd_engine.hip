@@ -258,14 +258,16 @@ __device__ inline int list_parity(const RepDev& R, uint64_t draws, int ahead) {
 }
 
 // Ermak-McCammon move of replica-local particle i (update_ermak.cu:91-160)
-// from (x, y, z) with force (fx, fy, fz) and normals (xi, yi, zi) at
-// temperature T: raw and wrapped positions written, the wrapped one inserted
+// from (x, y, z), raw position (ox, oy, oz), with force (fx, fy, fz) and
+// normals (xi, yi, zi) at temperature T: raw and wrapped positions written,
+// the wrapped one inserted
 // in its cell (do_bin).  With lm, the rebuild flag lflag[2 rep + par] is
 // raised for a move past half the skin.  Returns false for a non-finite
 // result.
 __device__ inline bool ermak_move(const RepDev& R, int rep, int i, double T, double x, double y,
                                   double z, double fx, double fy, double fz, double xi,
-                                  double yi, double zi, double* __restrict__ px,
+                                  double yi, double zi, double ox, double oy, double oz,
+                                  double* __restrict__ px,
                                   double* __restrict__ py, double* __restrict__ pz,
                                   double* __restrict__ rx, double* __restrict__ ry,
                                   double* __restrict__ rz, int* __restrict__ nocup,
@@ -282,7 +284,7 @@ __device__ inline bool ermak_move(const RepDev& R, int rep, int i, double T, dou
   x = x + dx;
   y = y + dy;
   z = z + dz;
-  const double qx = rx[p] + dx, qy = ry[p] + dy, qz = rz[p] + dz;
+  const double qx = ox + dx, qy = oy + dy, qz = oz + dz;
   rx[p] = qx;
   ry[p] = qy;
   rz[p] = qz;
@@ -331,15 +333,21 @@ __device__ __forceinline__ void k_update_bin_block(
   const size_t stride = (size_t)total_lanes;
   bool bad = false;
   for (int k = 0; k < R.per_lane; k++) {
+    const int i = l + k * R.lanes;
+    const bool live = i < R.n;
+    // the particle's state is loaded ahead of the draws, so that its memory
+    // latency overlaps theirs (a slot past the end reads particle 0)
+    const int p = R.p_off + (live ? i : 0);
+    const double x = px[p], y = py[p], z = pz[p];
+    const double gx = fx[p], gy = fy[p], gz = fz[p];
+    const double ox = rx[p], oy = ry[p], oz = rz[p];
     uint64_t d[4];
     draw4(rl, stride, np, lcg, d);
-    const int i = l + k * R.lanes;
-    if (i >= R.n) continue;
-    const int p = R.p_off + i;
+    if (!live) continue;
     double xi, yi, zi;
     bd::gauss3(d[0], d[1], d[2], d[3], xi, yi, zi);
-    if (!ermak_move(R, t.x, i, T, px[p], py[p], pz[p], fx[p], fy[p], fz[p], xi, yi, zi, px, py,
-                    pz, rx, ry, rz, nocup, ins, flags, do_bin, lm.lflag ? &lm : nullptr, par))
+    if (!ermak_move(R, t.x, i, T, x, y, z, gx, gy, gz, xi, yi, zi, ox, oy, oz, px, py, pz, rx,
+                    ry, rz, nocup, ins, flags, do_bin, lm.lflag ? &lm : nullptr, par))
       bad = true;
   }
   lcgs[gl] = lcg;
@@ -1301,7 +1309,9 @@ __device__ inline void fused_move(const RepDev& R, int rep, int i, double x, dou
                                   double zi, double* px, double* py, double* pz,
                                   const MoveArgs& mv) {
   const double T = temperature_at(R, *mv.step);
-  if (!ermak_move(R, rep, i, T, x, y, z, fx, fy, fz, xi, yi, zi, px, py, pz, mv.rx, mv.ry,
+  const int p = R.p_off + i;
+  if (!ermak_move(R, rep, i, T, x, y, z, fx, fy, fz, xi, yi, zi, mv.rx[p], mv.ry[p], mv.rz[p],
+                  px, py, pz, mv.rx, mv.ry,
                   mv.rz, mv.nocup, mv.ins, mv.flags, 1, mv.lm.lflag ? &mv.lm : nullptr,
                   list_parity(R, mv.draws[rep], 1)))
     atomicOr(&mv.flags[rep], kFlagNonFinite);
