@@ -495,6 +495,11 @@ int run_system() {
 }
 
 int main(int argc, char** argv) {
+  {
+    bd_params probe;  // an unknown BD_POTENTIAL ends the run before any work
+    std::memset(&probe, 0, sizeof probe);
+    set_potential_from_env(probe);
+  }
   if (argc > 1 && std::strcmp(argv[1], "--make-configs") == 0) return make_configs(argc, argv);
   if (argc > 1 && std::strcmp(argv[1], "--system") == 0) return run_system();
   // --aging: in log mode with decade dumps, also measure F_s and the MSD from

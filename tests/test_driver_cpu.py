@@ -38,6 +38,21 @@ def test_unknown_protocol_is_rejected(exe, tmp_path):
     assert r.returncode != 0
 
 
+def test_unknown_potential_is_rejected_at_start(exe, tmp_path):
+    env = dict(os.environ, BD_POTENTIAL="lennard_jones")
+    r = subprocess.run([exe], input=PROTOCOL, cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60, env=env)
+    assert r.returncode == 2
+    assert "unknown BD_POTENTIAL 'lennard_jones'" in r.stderr
+    assert "temp_protocol ?" not in r.stdout
+    # the 49-50 names are accepted: the run goes on to the protocol prompts
+    for name in ("wca", "wca_modified", "wca49_50", "wca49_50_ao"):
+        env = dict(os.environ, BD_POTENTIAL=name)
+        r = subprocess.run([exe], input=PROTOCOL, cwd=tmp_path, capture_output=True, text=True,
+                           timeout=60, env=env)
+        assert "temp_protocol ?" in r.stdout, name
+
+
 def test_run_aging_script_parses():
     import subprocess
     script = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
