@@ -407,6 +407,34 @@ def test_49_50_trajectory_100_steps(config1_positions, pot, kw):
     e.close()
 
 
+def test_49_50_replicas_and_persistent_steps(config1_positions):
+    """Two replicas with their own depletion constants in one engine, stepped by
+    the persistent kernel and then the graph, each equal to its own oracle run."""
+    side, x = config1_positions
+    params = [make_params(1000, 1000, side=side, T0=1.0, Tf=0.5, first_step=-5, seed=41,
+                          dt=1e-5, potential=BD_POT_WCA49_50_AO, **AO_FAST),
+              make_params(1000, 1000, side=side, T0=0.5, Tf=0.5, first_step=-5, seed=42,
+                          dt=1e-5, potential=BD_POT_WCA49_50_AO, sigma_pol=3.0, phi_pol=0.2)]
+    e = Engine(params)
+    for r in range(2):
+        e.set_positions(x, r)
+    e.set_persistent(16)
+    e.step(37)
+    e.set_persistent(0)
+    e.step(5)
+    for r in range(2):
+        o = orc.System(params[r])
+        o.set_positions(x)
+        assert o.cell_locate() == 0
+        o.compute_forces()
+        assert o.step(42) == 0
+        assert np.array_equal(e.positions(r), o.positions())
+        fe, ve, ue = e.forces(r)
+        fo, vo, uo = o.forces()
+        assert np.array_equal(fe, fo) and np.array_equal(ve, vo) and np.array_equal(ue, uo)
+    e.close()
+
+
 def test_quench_and_sine_protocols(config1_positions):
     side, x = config1_positions
     # instant quench at t0 = tf = 0 after a 20-step transient (first_step = -20)
