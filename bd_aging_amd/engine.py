@@ -33,6 +33,11 @@ def make_params(n_big, n_sml, side=None, phi=None, sigma_big=1.4, sigma_sml=1.0,
                 temp_protocol=abi.BD_TEMP_LINEAR, T0=1.0, Tf=1.0, t0=0.0, tf=0.0, period=1.0,
                 kappa=0.0, seed=12345, first_step=0, rng_lanes=0, pot_amp=None,
                 pot_inv_t=None, sigma_pol=0.0, phi_pol=0.0):
+    """bd_params of one system (encabezados.h:40-75).  Give the box edge `side`
+    or the packing fraction `phi`.  kappa: stiffness of BD_POT_HARMONIC.
+    pot_amp, pot_inv_t: A and 1/T of the 49-50 potentials (default: the
+    reference's commented values PHS_AMP, PHS_INV_T); sigma_pol, phi_pol: the
+    polymer diameter and volume fraction of BD_POT_WCA49_50_AO."""
     if side is None:
         if phi is None:
             raise ValueError("give side or phi")
