@@ -2348,6 +2348,50 @@ __global__ void __launch_bounds__(kOriginThreads)
 // atomics), then one global 64-bit atomic per bin per workgroup.
 constexpr int kGderThreads = 256;
 constexpr int kGderMaxBins = 512;
+
+// Minimum image of one axis (get_gder.cu:97-99): d - L floor(d / L + 0.5).
+// With U the image index K is the same for every pair of the wavefront and
+// tile, and sk = L K (the same product) is subtracted directly.
+template <bool U>
+__device__ inline double gder_image(double d, double side, double side_inv, double sk) {
+  if (U) return d - sk;
+  return d - side * floor(side_inv * d + 0.5);
+}
+
+template <int MASK>
+__device__ inline void gder_tile(const double* tx, const double* ty, const double* tz, int tn,
+                                 int jt, int i, double xi, double yi, double zi, double side,
+                                 double side_inv, double sx, double sy, double sz, double guard,
+                                 double range, double bin_size, int nbins, int si, int nb,
+                                 unsigned int* wh) {
+  for (int k = 0; k < tn; k++) {
+    const int j = jt + k;
+    if (j == i) continue;
+    const double dx = gder_image<(MASK & 1) != 0>(xi - tx[k], side, side_inv, sx);
+    const double dy = gder_image<(MASK & 2) != 0>(yi - ty[k], side, side_inv, sy);
+    const double dz = gder_image<(MASK & 4) != 0>(zi - tz[k], side, side_inv, sz);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (!(d2 < guard)) continue;
+    const double dist = sqrt(d2);
+    if (dist < range) {
+      int bin = (int)(dist / bin_size);
+      if (bin == nbins) bin--;
+      const int pr = 2 * si + (j < nb ? 0 : 1);
+      atomicAdd(&wh[pr * nbins + bin], 1u);
+    }
+  }
+}
+
+// Wavefront-wide min and max of v.
+__device__ inline void wave_minmax(double v_lo, double v_hi, double& lo, double& hi) {
+  lo = v_lo;
+  hi = v_hi;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, off, 64));
+    hi = fmax(hi, __shfl_xor(hi, off, 64));
+  }
+}
 __global__ void __launch_bounds__(kGderThreads)
     k_gder(const RepDev* __restrict__ reps, int rep, const double* __restrict__ px,
            const double* __restrict__ py, const double* __restrict__ pz, int nbins,
@@ -2382,6 +2426,13 @@ __global__ void __launch_bounds__(kGderThreads)
   unsigned int* wh = hist + (size_t)wave * 4 * nbins;
   const int j0 = (int)blockIdx.y * j_chunk;
   const int j1 = min(R.n, j0 + j_chunk);
+  // the wavefront's bounding box (its active particles)
+  const double inf = __builtin_inf();
+  double ilo[3], ihi[3];
+  wave_minmax(active ? xi : inf, active ? xi : -inf, ilo[0], ihi[0]);
+  wave_minmax(active ? yi : inf, active ? yi : -inf, ilo[1], ihi[1]);
+  wave_minmax(active ? zi : inf, active ? zi : -inf, ilo[2], ihi[2]);
+  const int lane = (int)threadIdx.x % 64;
   __syncthreads();
   for (int jt = j0; jt < j1; jt += kGderThreads) {
     const int jl = jt + (int)threadIdx.x;
@@ -2392,25 +2443,47 @@ __global__ void __launch_bounds__(kGderThreads)
     }
     __syncthreads();
     const int tn = min(kGderThreads, j1 - jt);
-    if (active) {
-      for (int k = 0; k < tn; k++) {
-        const int j = jt + k;
-        if (j == i) continue;
-        double dx = xi - tx[k];
-        double dy = yi - ty[k];
-        double dz = zi - tz[k];
-        dx -= side * floor(side_inv * dx + 0.5);
-        dy -= side * floor(side_inv * dy + 0.5);
-        dz -= side * floor(side_inv * dz + 0.5);
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        if (!(d2 < guard)) continue;
-        const double dist = sqrt(d2);
-        if (dist < range) {
-          int bin = (int)(dist / bin_size);
-          if (bin == nbins) bin--;
-          const int pr = 2 * si + (j < R.nb ? 0 : 1);
-          atomicAdd(&wh[pr * nbins + bin], 1u);
+    // Per axis: the image index K(d) = floor(RN(RN(d / L) + 0.5)) is
+    // monotonic in d, and every pair's d = RN(x_i - x_j) lies between
+    // RN(min x_i - max x_j) and RN(max x_i - min x_j); where both ends give
+    // the same K, so does every pair of this wavefront and tile.
+    int mask = 0;
+    double sh[3] = {0.0, 0.0, 0.0};
+    {
+      const double* tc[3] = {tx, ty, tz};
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        double lo = inf, hi = -inf;
+        for (int k = lane; k < tn; k += 64) {
+          lo = fmin(lo, tc[a][k]);
+          hi = fmax(hi, tc[a][k]);
         }
+        double jlo, jhi;
+        wave_minmax(lo, hi, jlo, jhi);
+        const double kmin = floor(side_inv * (ilo[a] - jhi) + 0.5);
+        const double kmax = floor(side_inv * (ihi[a] - jlo) + 0.5);
+        if (kmin == kmax) {
+          mask |= 1 << a;
+          sh[a] = side * kmin;
+        }
+      }
+    }
+    if (active) {
+      switch (mask) {
+#define BD_GDER_CASE(M)                                                                        \
+  case M:                                                                                      \
+    gder_tile<M>(tx, ty, tz, tn, jt, i, xi, yi, zi, side, side_inv, sh[0], sh[1], sh[2], guard, \
+                 range, bin_size, nbins, si, R.nb, wh);                                        \
+    break;
+        BD_GDER_CASE(0)
+        BD_GDER_CASE(1)
+        BD_GDER_CASE(2)
+        BD_GDER_CASE(3)
+        BD_GDER_CASE(4)
+        BD_GDER_CASE(5)
+        BD_GDER_CASE(6)
+        BD_GDER_CASE(7)
+#undef BD_GDER_CASE
       }
     }
     __syncthreads();

@@ -544,6 +544,45 @@ def test_gder_counts_bit_exact(config1_positions):
     e.close()
 
 
+def morton_order(x, side, bits=6):
+    """Indices sorting positions along a Z-order curve of 2^bits cells per axis."""
+    c = np.minimum((x / side * (1 << bits)).astype(np.int64), (1 << bits) - 1)
+    key = np.zeros(len(x), dtype=np.int64)
+    for b in range(bits):
+        for a in range(3):
+            key |= ((c[:, a] >> b) & 1) << (3 * b + a)
+    return np.argsort(key, kind="stable")
+
+
+def test_gder_uniform_image_tiles_bit_exact():
+    """N=16384 in Z order (a wavefront's 64 particles and a tile's 256 are
+    compact): most wavefront/tile blocks take the per-axis uniform image
+    shortcut of k_gder, and the counts still equal the oracle's."""
+    nb = ns = 8192
+    p = make_params(nb, ns, phi=0.5)
+    side = p.side
+    rng = np.random.default_rng(77)
+    x = rng.random((nb + ns, 3)) * side
+    x[:nb] = x[:nb][morton_order(x[:nb], side)]
+    x[nb:] = x[nb:][morton_order(x[nb:], side)]
+    # how many (wavefront, tile) blocks have one image index per axis, by the
+    # kernel's rule (64 consecutive i, 256 consecutive j)
+    inv = 1.0 / side
+    w = x.reshape(-1, 64, 3)
+    t = x.reshape(-1, 256, 3)
+    wlo, whi, tlo, thi = w.min(1), w.max(1), t.min(1), t.max(1)
+    kmin = np.floor(inv * (wlo[:, None, :] - thi[None, :, :]) + 0.5)
+    kmax = np.floor(inv * (whi[:, None, :] - tlo[None, :, :]) + 0.5)
+    uniform = kmin == kmax
+    assert uniform.mean() > 0.3 and uniform.any(axis=2).mean() > 0.6
+    e, o = pair(p, x)
+    for nbins, rg in [(128, 5.0), (200, side / 2)]:
+        ce = e.gder_counts(nbins, rg)
+        co = o.gder_counts(nbins, rg)
+        assert np.array_equal(ce, co), (nbins, rg)
+    e.close()
+
+
 def test_observables_close(config1_positions):
     side, x = config1_positions
     p = make_params(1000, 1000, side=side, T0=0.8, Tf=0.8, seed=3)
