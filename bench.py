@@ -103,8 +103,9 @@ def run_plain(steps, warmup, graph_steps=16, shape=None):
     t0 = time.perf_counter()
     reps = 5
     for _ in range(reps):
-        eng.gder_counts(128, 5.0)
+        gcounts = eng.gder_counts(128, 5.0)
     gder_s = (time.perf_counter() - t0) / reps
+    gder_hits = int(np.asarray(gcounts).sum())
     # force contraction (Contracting_Vector3D): one sample of the current state
     cv = Contraction(CV3D_BINS, CV3D_RANGE)
     cv.add(eng)  # warm-up
@@ -116,7 +117,7 @@ def run_plain(steps, warmup, graph_steps=16, shape=None):
     eng.close()
     return {"N": nb + ns, "side": side, "seconds": dt, "rate": (nb + ns) * steps / dt,
             "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p, "shape": shape,
-            "gder_s": gder_s, "fused_ms": fused_ms, "cv3d_s": cv3d_s}
+            "gder_s": gder_s, "gder_hits": gder_hits, "fused_ms": fused_ms, "cv3d_s": cv3d_s}
 
 
 def run_batch(steps, warmup, graph_steps=16, shape=None):
@@ -170,8 +171,9 @@ def run_stress(steps, warmup, graph_steps=16, shape=None):
     geo = eng.geometry()
     eng.gder_counts(128, 5.0)  # warm-up
     t0 = time.perf_counter()
-    eng.gder_counts(128, 5.0)
+    gcounts = eng.gder_counts(128, 5.0)
     gder_s = time.perf_counter() - t0
+    gder_hits = int(np.asarray(gcounts).sum())
     eng.close()
     N = nb + ns
     kern = {}
@@ -187,6 +189,8 @@ def run_stress(steps, warmup, graph_steps=16, shape=None):
             "gder": {"pairs_per_s": N * (N - 1) / gder_s,
                      "GFLOP_s": FLOP_PAIR * N * (N - 1) / gder_s / 1e9,
                      "fp64_frac": FLOP_PAIR * N * (N - 1) / gder_s / FP64_PEAK,
+                     # one LDS atomic increment per ordered pair within range
+                     "lds_atomics_per_s": gder_hits / gder_s,
                      "bins": 128, "range": 5.0}}
 
 
@@ -358,6 +362,7 @@ def main():
         "gder": {"pairs_per_s": N * (N - 1) / plain["gder_s"],
                  "GFLOP_s": FLOP_PAIR * N * (N - 1) / plain["gder_s"] / 1e9,
                  "fp64_frac": FLOP_PAIR * N * (N - 1) / plain["gder_s"] / FP64_PEAK,
+                 "lds_atomics_per_s": plain["gder_hits"] / plain["gder_s"],
                  "flop_per_pair": FLOP_PAIR, "bins": 128, "range": 5.0},
         "cv3d": {"bins": CV3D_BINS, "range": CV3D_RANGE, "ms_per_sample": plain["cv3d_s"] * 1e3,
                  "GFLOP_s": cv3d_flop / plain["cv3d_s"] / 1e9,
