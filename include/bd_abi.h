@@ -179,7 +179,9 @@ int bd_get_persistent(const bd_sys* s, int32_t* steps);
 /* Force-kernel shape (no reference counterpart; the autotuner's knobs):
  * brick = 1..4 selects the LDS-staged kernel over brick^3 big-grid cells,
  * 0 the thread-per-particle row walk.  Every choice gives bit-identical
- * results. */
+ * results.  The 49-50 potentials (BD_POT_WCA49_50, _AO) run the slab walk
+ * at deferral depth 16 whatever the shape settings (bricks, walks, pass
+ * split, fused update) say; the settings are kept and reported. */
 int bd_set_brick(bd_sys* s, int32_t brick);
 int bd_get_brick(const bd_sys* s, int32_t* brick);
 
@@ -268,7 +270,8 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
  * BD_ORDER_SPLIT4 / _SPLIT8: 4 or 8 lanes share a particle's stencil and
  *   their partial sums are combined in a fixed order: deterministic, equal to
  *   the reference order to rounding (~1e-15 relative), faster on small
- *   systems.  Needs grids of at least 2*nrange+3 cells. */
+ *   systems.  Needs grids of at least 2*nrange+3 cells; BD_E_ARG with the
+ *   49-50 potentials, which run in the reference order only. */
 enum { BD_ORDER_REFERENCE = 1, BD_ORDER_SPLIT4 = 4, BD_ORDER_SPLIT8 = 8 };
 int bd_set_force_order(bd_sys* s, int32_t order);
 int bd_get_force_order(const bd_sys* s, int32_t* order);
