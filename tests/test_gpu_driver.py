@@ -36,26 +36,39 @@ def read_config(path):
     return nb, ns, side, sb, ss, xyz
 
 
-def test_driver_matches_oracle_replay(tmp_path):
+@pytest.mark.parametrize("pot_name,pot,dt", [("wca_modified", abi.BD_POT_WCA_MODIFIED, 1e-4),
+                                             ("wca49_50", abi.BD_POT_WCA49_50, 1e-5)])
+def test_driver_matches_oracle_replay(tmp_path, pot_name, pot, dt):
+    """BD_POTENTIAL selects the pair potential of the configurations'
+    equilibration and of the run.  The 49-50 potential takes dt = 1e-5, its
+    wall being too stiff for explicit steps of 1e-4; its configurations are
+    equilibrated with it (wca_modified leaves pairs at 0.9 sigma at T = 1,
+    which the s^50 wall would throw apart)."""
     cfg = tmp_path / "configs"
     sim = tmp_path / "sim"
     (sim / "results").mkdir(parents=True)
+    env = dict(os.environ, BD_POTENTIAL=pot_name)
     r = subprocess.run([EXE, "--make-configs", "250", "250", "0.45", "1.4", "1.0", "2", "77",
-                        str(cfg), "1.0", "50", "1e-4"], capture_output=True, text=True, timeout=120)
+                        str(cfg), "1.0", "50", f"{dt:g}"], capture_output=True, text=True,
+                       timeout=120, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
-    r = subprocess.run([EXE], input=PROTOCOL, cwd=sim, capture_output=True, text=True,
-                       timeout=120)
+    protocol = PROTOCOL
+    if dt != 1e-4:  # the same 20 + 60 steps: (int)(time / dt) of the driver
+        protocol = protocol.replace("\n1e-4\n", f"\n{dt:g}\n").replace(
+            "0.002  0.006  3", f"{20.1 * dt:.6g}  {60.1 * dt:.6g}  3")
+    r = subprocess.run([EXE], input=protocol, cwd=sim, capture_output=True, text=True,
+                       timeout=120, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "TERMINADO" in r.stdout
     # replay on the oracle
-    dt, ntrans, niter, nsamples, nbins, rng, q = 1e-4, 20, 60, 3, 40, 4.0, 7.0
+    ntrans, niter, nsamples, nbins, rng, q = 20, 60, 3, 40, 4.0, 7.0
     samples = list(range(0, niter, niter // nsamples))
     acc = {k: np.zeros(len(samples)) for k in ("fs", "mb", "ms", "eb", "es", "z", "T")}
     gsum = np.zeros((len(samples), 4, nbins))
     for i in range(2):
         nb, ns, side, sb, ss, xyz = read_config(cfg / f"init_config_{i + 1}")
         p = make_params(nb, ns, side=side, sigma_big=sb, sigma_sml=ss, dt=dt,
-                        potential=abi.BD_POT_WCA_MODIFIED, T0=1.0, Tf=0.1, t0=0.0, tf=0.0,
+                        potential=pot, T0=1.0, Tf=0.1, t0=0.0, tf=0.0,
                         seed=lane_seed(4321, i), first_step=-ntrans)
         o = orc.System(p)
         o.set_positions(xyz)
