@@ -680,6 +680,25 @@ def test_autotune_keeps_results_and_caches(config1_positions, tmp_path):
     e2.close()
 
 
+def test_autotune_under_49_50_times_its_one_kernel(config1_positions, tmp_path):
+    """The 49-50 potentials run one force kernel whatever the shape: the
+    autotuner times that shape only (slab walk, depth 16) and caches it."""
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, seed=3, potential=BD_POT_WCA49_50_AO, **AO_FAST)
+    e = Engine(p)
+    e.set_positions(x)
+    e.cell_locate()
+    e.compute_forces()
+    f0 = e.forces()
+    cache = str(tmp_path / "tune.txt")
+    assert e.autotune(3, cache) == 0
+    assert (e.walk(), e.defer_depth(), e.pass_split()) == (abi.BD_WALK_SLABS, 16, 0)
+    e.compute_forces()
+    assert all(np.array_equal(a, c) for a, c in zip(f0, e.forces()))
+    assert open(cache).read().split()[-4:] == ["0", "16", str(abi.BD_WALK_SLABS), "0"]
+    e.close()
+
+
 # ------------------------------------------------------------------ checkpoint
 def test_checkpoint_resume_bit_exact(config1_positions, tmp_path):
     side, x = config1_positions
