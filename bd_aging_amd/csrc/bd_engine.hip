@@ -2568,6 +2568,7 @@ struct bd_sys {
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;       // steps in the instantiated graph
+  bool graph_fused = false;  // the graph holds fused (forces + move, cells) pairs
   int graph_mode = 16;       // steps per graph launch; 0 = eager
   int persist = 0;           // steps per persistent launch; 0 = off
   int persist_grid = 0;      // co-resident workgroups of the persistent kernel
@@ -3697,15 +3698,24 @@ int bd_update(bd_sys* s) {
   return rc;
 }
 
-// Capture `steps` steps into one executable graph.
-static int build_graph(bd_sys* s, int steps) {
+// Capture `steps` steps into one executable graph: whole steps, or with
+// `fused` the steady state of the fused update, `steps` pairs of (forces of
+// the current positions + the next move, binning).
+static int build_graph(bd_sys* s, int steps, bool fused) {
   if (s->graph) {
     HIP_TRY(hipGraphExecDestroy(s->graph));
     s->graph = nullptr;
   }
   hipGraph_t g = nullptr;
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_steps(s, steps);
+  if (fused) {
+    for (int j = 0; j < steps; j++) {
+      launch_forces_move(s);
+      launch_cells(s, 1);
+    }
+  } else {
+    enqueue_steps(s, steps);
+  }
   HIP_TRY(hipStreamEndCapture(s->stream, &g));
   hipError_t e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -3714,6 +3724,7 @@ static int build_graph(bd_sys* s, int steps) {
     return fail(BD_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
   s->graph_steps = steps;
+  s->graph_fused = fused;
   return BD_OK;
 }
 
@@ -3744,10 +3755,31 @@ int bd_step(bd_sys* s, int32_t nsteps) {
     return rc == BD_OK ? list_invalidate(s) : rc;
   }
   const int G = s->graph_mode;
+  if (G > 0 && nsteps > G && fusable(s)) {
+    // the kernels of enqueue_steps(s, nsteps) in one run: the first step's
+    // update, nsteps - 1 fused pairs (graph-replayed G at a time), the forces
+    // of the final positions
+    launch_update(s, 1);
+    launch_cells(s, 1);
+    int pairs = nsteps - 1;
+    if (!s->graph || s->graph_steps != G || !s->graph_fused) {
+      rc = build_graph(s, G, true);
+      if (rc != BD_OK) return rc;
+    }
+    for (; pairs >= G; pairs -= G) HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+    for (; pairs > 0; pairs--) {
+      launch_forces_move(s);
+      launch_cells(s, 1);
+    }
+    launch_forces(s);
+    HIP_TRY(hipGetLastError());
+    s->ni += nsteps;
+    return check_flags(s);
+  }
   int left = nsteps;
   if (G > 0 && left >= G) {
-    if (!s->graph || s->graph_steps != G) {
-      rc = build_graph(s, G);
+    if (!s->graph || s->graph_steps != G || s->graph_fused) {
+      rc = build_graph(s, G, false);
       if (rc != BD_OK) return rc;
     }
     while (left >= G) {

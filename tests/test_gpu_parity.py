@@ -234,7 +234,8 @@ def test_fused_update_bit_identical(config1_positions, split):
         e.set_fused_update(fused)
         assert e.fused_update() == fused
         e.set_positions(x)
-        e.step(37)  # two 16-step graphs and 5 eager steps
+        e.step(37)  # unfused: two 16-step graphs, 5 eager steps; fused: the first update,
+        #             two graphs of 16 fused pairs, 4 eager pairs, the last forces
         runs.append([e.positions(), e.raw_positions(), *e.forces(), *e.rng_lane(5),
                      *e.rng_lane(1999), *e.cells(0), *e.cells(1)])
         e.close()
