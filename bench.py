@@ -284,9 +284,11 @@ def main():
     ap.add_argument("--shape", default=None,
                     help="force kernel as BRICK,DEPTH[,WALK[,SPLIT]] (e.g. 0,16,3,2) instead of "
                          "autotuning")
-    ap.add_argument("--graph-steps", type=int, default=16,
-                    help="steps per captured HIP graph; 0 launches every kernel eagerly "
-                         "(same kernels, for kernel tracing)")
+    ap.add_argument("--graph-steps", type=int, default=256,
+                    help="steps per captured HIP graph (the engine's default is 16; the "
+                         "timed runs here are long enough for 256, which leaves fewer gaps "
+                         "between graph launches); 0 launches every kernel eagerly (same "
+                         "kernels, for kernel tracing)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the runs, copy a 1 GiB buffer 3 times (a plain copy kernel of "
                          "known traffic, the scale for FETCH_SIZE / WRITE_SIZE counters)")
