@@ -3759,13 +3759,14 @@ int bd_step(bd_sys* s, int32_t nsteps) {
     // the kernels of enqueue_steps(s, nsteps) in one run: the first step's
     // update, nsteps - 1 fused pairs (graph-replayed G at a time), the forces
     // of the final positions
-    launch_update(s, 1);
-    launch_cells(s, 1);
-    int pairs = nsteps - 1;
+    // (the graph is built first, so that a failure leaves no step half taken)
     if (!s->graph || s->graph_steps != G || !s->graph_fused) {
       rc = build_graph(s, G, true);
       if (rc != BD_OK) return rc;
     }
+    launch_update(s, 1);
+    launch_cells(s, 1);
+    int pairs = nsteps - 1;
     for (; pairs >= G; pairs -= G) HIP_TRY(hipGraphLaunch(s->graph, s->stream));
     for (; pairs > 0; pairs--) {
       launch_forces_move(s);
