@@ -162,8 +162,11 @@ int bd_step(bd_sys* s, int32_t nsteps);
 
 /* Launch mode of bd_step (no reference counterpart): steps per captured
  * HIP graph, 1..256 (default 16), or 0 to enqueue every kernel eagerly (for
- * tracing tools that do not follow graph launches).  Results are
- * bit-identical in every mode. */
+ * tracing tools that do not follow graph launches).  A call shorter than a
+ * graph runs eagerly.  With the fused update the graph holds that many
+ * (forces + next move, binning) pairs, and the call's first update and last
+ * forces are launched once around the replays.  Results are bit-identical
+ * in every mode. */
 int bd_set_graph_steps(bd_sys* s, int32_t steps);
 int bd_get_graph_steps(const bd_sys* s, int32_t* steps);
 
