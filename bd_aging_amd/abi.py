@@ -96,6 +96,7 @@ SIGNATURES = {
     "bd_neighbours": (C.c_int, [_P, C.c_int32, C.c_int32, _I, _I]),
     "bd_update": (C.c_int, [_P]),
     "bd_step": (C.c_int, [_P, C.c_int32]),
+    "bd_prepare_step": (C.c_int, [_P, C.c_int32]),
     "bd_autotune": (C.c_int, [_P, C.c_int32, C.c_char_p, _I]),
     "bd_set_force_order": (C.c_int, [_P, C.c_int32]),
     "bd_get_force_order": (C.c_int, [_P, _I]),

@@ -7,6 +7,7 @@ the kernels evaluate the same IEEE operations as the FP64 oracle;
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -31,9 +32,17 @@ def _run(cmd, verbose):
 
 
 def build_lib(verbose=True):
+    """Each source compiled to an object concurrently, then one link."""
     out = os.path.join(HERE, "libbd_hip.so")
-    srcs = [os.path.join(CSRC, s) for s in LIB_SOURCES]
-    _run([HIPCC] + COMMON + ["-fPIC", "-shared", "-o", out] + srcs, verbose)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in LIB_SOURCES]
+    cmds = [[HIPCC] + COMMON + ["-fPIC", "-c", "-o", o, os.path.join(CSRC, s)]
+            for s, o in zip(LIB_SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
+    _run([HIPCC, f"--offload-arch={ARCH}", "-fPIC", "-shared", "-o", out] + objs, verbose)
     return out
 
 

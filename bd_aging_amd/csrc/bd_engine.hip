@@ -93,6 +93,20 @@ struct RepDev {
   uint64_t seed;
 };
 
+// Error record of one replica: flag bits (kFlag*) and the step index ni of
+// the first update (or initial binning) whose result raised one.
+struct RepFlags {
+  int bits;
+  int pad_;
+  long long first_step;
+};
+constexpr long long kNoFailStep = 0x7fffffffffffffffLL;
+
+__device__ inline void raise_flag(RepFlags* f, int rep, int bit, int64_t ni) {
+  atomicOr(&f[rep].bits, bit);
+  atomicMin(&f[rep].first_step, (long long)ni);
+}
+
 thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
@@ -152,20 +166,23 @@ __device__ inline bool cell_coords(const RepDev& R, int g, double x, double y, d
 }
 
 __device__ inline void insert_cell(const RepDev& R, int rep, int i, double x, double y,
-                                   double z, int* nocup, int* ins, int* flags) {
+                                   double z, int* nocup, int* ins, RepFlags* flags,
+                                   int64_t ni) {
   const int g = i < R.nb ? 0 : 1;
   const int loc = g == 0 ? i : i - R.nb;
   int ii, jj, kk, io, jo, ko;
   // a position the force walk cannot place in both grids (non-finite, or far
   // outside the box) is an error of the whole replica
   if (!cell_coords(R, g, x, y, z, ii, jj, kk) || !cell_coords(R, 1 - g, x, y, z, io, jo, ko)) {
-    atomicOr(&flags[rep], kFlagNonFinite);
+    raise_flag(flags, rep, kFlagNonFinite, ni);
     return;
   }
   const int n = R.ncell[g];
   const int c = kk + n * (ii + n * jj);  // z fastest: a row's cells are adjacent
   const int slot = atomicAdd(&nocup[R.cell_off[g] + c], 1);
   if (slot < R.ntags[g]) ins[(size_t)R.tag_off[g] + (size_t)c * R.ntags[g] + slot] = loc;
+  // the insertion that fills the cell to ntags (the main.cu:655-659 rule)
+  if (slot >= R.ntags[g] - 1) raise_flag(flags, rep, kFlagOverflow, ni);
 }
 
 // ---------------------------------------------------------------------------
@@ -197,14 +214,14 @@ __global__ void __launch_bounds__(kBlk)
     k_bin(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
           const double* __restrict__ px, const double* __restrict__ py,
           const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ ins,
-          int* __restrict__ flags) {
+          RepFlags* __restrict__ flags, int64_t ni) {
   const int4 t = tbl[blockIdx.x];
   if (t.x < 0) return;
   const int i = t.y + (int)threadIdx.x;
   if (i >= t.z) return;
   const RepDev& R = reps[t.x];
   const int p = R.p_off + i;
-  insert_cell(R, t.x, i, px[p], py[p], pz[p], nocup, ins, flags);
+  insert_cell(R, t.x, i, px[p], py[p], pz[p], nocup, ins, flags, ni);
 }
 
 // Four rannew64 draws of one lane (rannew64.cu:99-126): ring slot np and
@@ -271,8 +288,8 @@ __device__ inline bool ermak_move(const RepDev& R, int rep, int i, double T, dou
                                   double* __restrict__ py, double* __restrict__ pz,
                                   double* __restrict__ rx, double* __restrict__ ry,
                                   double* __restrict__ rz, int* __restrict__ nocup,
-                                  int* __restrict__ ins, int* __restrict__ flags, int do_bin,
-                                  const ListMove* lm = nullptr, int par = 0) {
+                                  int* __restrict__ ins, RepFlags* __restrict__ flags, int do_bin,
+                                  int64_t ni, const ListMove* lm = nullptr, int par = 0) {
   const int p = R.p_off + i;
   const double side = R.side, side_inv = R.side_inv, dt = R.dt;
   const double diameter = i < R.nb ? R.sigma[0] : R.sigma[1];
@@ -298,7 +315,7 @@ __device__ inline bool ermak_move(const RepDev& R, int rep, int i, double T, dou
   px[p] = x;
   py[p] = y;
   pz[p] = z;
-  if (do_bin) insert_cell(R, rep, i, x, y, z, nocup, ins, flags);
+  if (do_bin) insert_cell(R, rep, i, x, y, z, nocup, ins, flags, ni);
   return isfinite(x) && isfinite(y) && isfinite(z);
 }
 
@@ -316,7 +333,7 @@ __device__ __forceinline__ void k_update_bin_block(
                  const double* __restrict__ fz, uint64_t* __restrict__ ring,
                  uint64_t* __restrict__ lcgs, const uint64_t* __restrict__ draws,
                  const int64_t* __restrict__ step, int total_lanes, int* __restrict__ nocup,
-                 int* __restrict__ ins, int* __restrict__ flags, int do_bin,
+                 int* __restrict__ ins, RepFlags* __restrict__ flags, int do_bin,
                  ListMove lm = ListMove{nullptr, nullptr, nullptr, nullptr, 0.0}) {
   const int4 t = tbl[vb];
   if (t.x < 0) return;
@@ -324,7 +341,8 @@ __device__ __forceinline__ void k_update_bin_block(
   if (l >= t.z) return;
   const RepDev& R = reps[t.x];
   const int gl = R.lane_off + l;
-  const double T = temperature_at(R, *step);
+  const int64_t ni = *step;
+  const double T = temperature_at(R, ni);
   const uint64_t dr = draws[t.x];
   const int par = list_parity(R, dr, 1);
   uint64_t lcg = lcgs[gl];
@@ -347,11 +365,11 @@ __device__ __forceinline__ void k_update_bin_block(
     double xi, yi, zi;
     bd::gauss3(d[0], d[1], d[2], d[3], xi, yi, zi);
     if (!ermak_move(R, t.x, i, T, x, y, z, gx, gy, gz, xi, yi, zi, ox, oy, oz, px, py, pz, rx,
-                    ry, rz, nocup, ins, flags, do_bin, lm.lflag ? &lm : nullptr, par))
+                    ry, rz, nocup, ins, flags, do_bin, ni, lm.lflag ? &lm : nullptr, par))
       bad = true;
   }
   lcgs[gl] = lcg;
-  if (bad) atomicOr(&flags[t.x], kFlagNonFinite);
+  if (bad) raise_flag(flags, t.x, kFlagNonFinite, ni);
 }
 
 // The normals of particle i's next move, for a replica whose lanes are its
@@ -378,7 +396,7 @@ __global__ void __launch_bounds__(kBlk)
                  const double* __restrict__ fz, uint64_t* __restrict__ ring,
                  uint64_t* __restrict__ lcgs, const uint64_t* __restrict__ draws,
                  const int64_t* __restrict__ step, int total_lanes, int* __restrict__ nocup,
-                 int* __restrict__ ins, int* __restrict__ flags, int do_bin, ListMove lm) {
+                 int* __restrict__ ins, RepFlags* __restrict__ flags, int do_bin, ListMove lm) {
   k_update_bin_block((int)blockIdx.x, reps, tbl, px, py, pz, rx, ry, rz, fx, fy, fz, ring, lcgs,
                      draws, step, total_lanes, nocup, ins, flags, do_bin, lm);
 }
@@ -402,7 +420,7 @@ __device__ __forceinline__ void k_cells_block(
             const double* __restrict__ px, const double* __restrict__ py,
             const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
             const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
-            int* __restrict__ flags, int64_t* __restrict__ step,
+            RepFlags* __restrict__ flags, int64_t* __restrict__ step,
             uint64_t* __restrict__ draws, int advance, int* __restrict__ lflag = nullptr) {
   const int4 t = tbl[vb];
   if (advance && threadIdx.x == 0) {
@@ -496,7 +514,7 @@ __device__ __forceinline__ void k_cells_block(
     carry += row_sum;
   }
   if (row_live && zl == 0) rrow[n] = row_base + carry;
-  if (over) atomicOr(&flags[t.x], kFlagOverflow);
+  if (over) atomicOr(&flags[t.x].bits, kFlagOverflow);
 }
 
 __global__ void __launch_bounds__(kBlk)
@@ -504,7 +522,7 @@ __global__ void __launch_bounds__(kBlk)
             const double* __restrict__ px, const double* __restrict__ py,
             const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
             const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
-            int* __restrict__ flags, int64_t* __restrict__ step,
+            RepFlags* __restrict__ flags, int64_t* __restrict__ step,
             uint64_t* __restrict__ draws, int advance, int* __restrict__ lflag) {
   k_cells_block((int)blockIdx.x, reps, tbl, px, py, pz, nocup, cnt, ins, rs, cpos, flags, step,
                 draws, advance, lflag);
@@ -1298,7 +1316,8 @@ struct MoveArgs {
   const int64_t* step;
   int total_lanes;
   double *rx, *ry, *rz;
-  int *nocup, *ins, *flags;
+  int *nocup, *ins;
+  RepFlags* flags;
   ListMove lm;  // lm.lflag == nullptr: no neighbour lists
 };
 
@@ -1308,13 +1327,14 @@ __device__ inline void fused_move(const RepDev& R, int rep, int i, double x, dou
                                   double fx, double fy, double fz, double xi, double yi,
                                   double zi, double* px, double* py, double* pz,
                                   const MoveArgs& mv) {
-  const double T = temperature_at(R, *mv.step);
+  const int64_t ni = *mv.step;
+  const double T = temperature_at(R, ni);
   const int p = R.p_off + i;
   if (!ermak_move(R, rep, i, T, x, y, z, fx, fy, fz, xi, yi, zi, mv.rx[p], mv.ry[p], mv.rz[p],
                   px, py, pz, mv.rx, mv.ry,
-                  mv.rz, mv.nocup, mv.ins, mv.flags, 1, mv.lm.lflag ? &mv.lm : nullptr,
+                  mv.rz, mv.nocup, mv.ins, mv.flags, 1, ni, mv.lm.lflag ? &mv.lm : nullptr,
                   list_parity(R, mv.draws[rep], 1)))
-    atomicOr(&mv.flags[rep], kFlagNonFinite);
+    raise_flag(mv.flags, rep, kFlagNonFinite, ni);
 }
 
 template <int POT, bool NEIGH, int DEPTH, int WALK, bool UPD = false>
@@ -1919,7 +1939,8 @@ struct StepArgs {
   double *px, *py, *pz, *rx, *ry, *rz, *fx, *fy, *fz, *vir, *pot;
   uint64_t *ring, *lcg, *draws;
   int64_t* step;
-  int *nocup, *cnt, *ins, *rs, *flags;
+  int *nocup, *cnt, *ins, *rs;
+  RepFlags* flags;
   double4* cpos;
 };
 
@@ -2051,7 +2072,7 @@ __global__ void __launch_bounds__(kBrickThreads)
                    const double* __restrict__ pz, const int* __restrict__ rs,
                    const double4* __restrict__ cpos,
                    double* __restrict__ ffx, double* __restrict__ ffy, double* __restrict__ ffz,
-                   double* __restrict__ vir, double* __restrict__ pot, int* __restrict__ flags) {
+                   double* __restrict__ vir, double* __restrict__ pot, RepFlags* __restrict__ flags) {
   __shared__ unsigned int tbl[kBrickMaxCells];
   __shared__ double4 spos[CAP];
   __shared__ int home[CAP];
@@ -2084,7 +2105,7 @@ __global__ void __launch_bounds__(kBrickThreads)
   const int nbr = bb.dim[0] * bb.dim[1];
   const int nrows = nbr + sb.dim[0] * sb.dim[1];
   if (ncells > kBrickMaxCells || nrows > kBrickMaxRows) {  // excluded on the host
-    if (tid == 0) atomicOr(&flags[t.x], kFlagBrick);
+    if (tid == 0) atomicOr(&flags[t.x].bits, kFlagBrick);
     return;
   }
   // 1. occupancy of every staged cell, one z-run (packed row) per thread
@@ -2199,7 +2220,7 @@ __global__ void __launch_bounds__(kBrickThreads)
     if (pass == 0) {
       const int nh = block_scan(rows, nhr, part);
       if (nh > CAP) {  // cannot happen below 32 particles per cell
-        if (tid == 0) atomicOr(&flags[t.x], kFlagBrick);
+        if (tid == 0) atomicOr(&flags[t.x].bits, kFlagBrick);
         return;
       }
       if (tid == 0) part[0] = nh;  // keep the count (part is free after the scan)
@@ -2562,13 +2583,16 @@ struct bd_sys {
   double4* cpos = nullptr;
   uint64_t *ring = nullptr, *lcg = nullptr, *draws = nullptr;
   int64_t* step = nullptr;
-  int* flags = nullptr;
+  RepFlags* flags = nullptr;
   int pot_kind = BD_POT_WCA;
   bool located = false;
   hipStream_t stream = nullptr;
-  hipGraphExec_t graph = nullptr;
-  int graph_steps = 0;       // steps in the instantiated graph
-  bool graph_fused = false;  // the graph holds fused (forces + move, cells) pairs
+  // cached step graphs: whole steps, and the fused path's (forces + next
+  // move, binning) pairs; each kept until a setting or the length changes
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    int steps = 0;
+  } graph_plain, graph_fused;
   int graph_mode = 16;       // steps per graph launch; 0 = eager
   int persist = 0;           // steps per persistent launch; 0 = off
   int persist_grid = 0;      // co-resident workgroups of the persistent kernel
@@ -2604,6 +2628,11 @@ struct bd_sys {
   int4* tbl_split = nullptr;
   // waiting-time origins: BD_MAX_ORIGINS slots of raw positions (x, y, z)
   double* origins = nullptr;
+  // device buffer of the measurement entry points' per-call results (grown
+  // on demand, then reused: a sampling loop allocates nothing after its
+  // first call)
+  void* callbuf = nullptr;
+  size_t callbuf_bytes = 0;
 };
 
 namespace {
@@ -2632,7 +2661,7 @@ void launch(bd_sys* s, void (*k)(P...), int nblk, int nthr, A... a) {
 int launch_bin(bd_sys* s) {
   const int nb = (int)s->tbl_part_h.size();
   launch(s, k_bin, nb, kBlk, s->reps_d, s->tbl_part, s->px, s->py, s->pz, s->nocup,
-                                    s->ins, s->flags);
+                                    s->ins, s->flags, (int64_t)s->ni);
   return BD_OK;
 }
 
@@ -3068,21 +3097,68 @@ void enqueue_steps(bd_sys* s, int k) {
   launch_forces(s);
 }
 
+// Error records after the enqueued work: the first replica with a flag
+// raised fails the call, naming the step whose update (or binning) raised it.
+// A record stays set until that replica's state is replaced
+// (bd_set_positions, bd_checkpoint_load).
 int check_flags(bd_sys* s) {
-  std::vector<int> f(s->R);
-  HIP_TRY(hipMemcpyAsync(f.data(), s->flags, sizeof(int) * s->R, hipMemcpyDeviceToHost,
+  std::vector<RepFlags> f(s->R);
+  HIP_TRY(hipMemcpyAsync(f.data(), s->flags, sizeof(RepFlags) * s->R, hipMemcpyDeviceToHost,
                          s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   for (int r = 0; r < s->R; r++) {
-    if (f[r] & kFlagNonFinite)
-      return fail(BD_E_NAN, "replica " + std::to_string(r) +
-                                ": non-finite position, or one outside the cell grids");
-    if (f[r] & kFlagBrick)
-      return fail(BD_E_STATE, "replica " + std::to_string(r) + ": brick exceeds its LDS table");
-    if (f[r] & kFlagOverflow)
-      return fail(BD_E_CELL_OVERFLOW,
-                  "replica " + std::to_string(r) + ": a cell reached ntags particles");
+    const std::string where =
+        "replica " + std::to_string(r) +
+        (f[r].first_step != kNoFailStep ? " at step " + std::to_string(f[r].first_step) : "");
+    if (f[r].bits & kFlagNonFinite)
+      return fail(BD_E_NAN, where + ": non-finite position, or one outside the cell grids");
+    if (f[r].bits & kFlagBrick)
+      return fail(BD_E_STATE, where + ": brick exceeds its LDS table");
+    if (f[r].bits & kFlagOverflow)
+      return fail(BD_E_CELL_OVERFLOW, where + ": a cell reached ntags particles");
   }
+  return BD_OK;
+}
+
+// Clears the error records of replicas [r0, r1) (queued on the engine's
+// stream, before any later launch).
+int clear_flags(bd_sys* s, int r0, int r1) {
+  if (r1 <= r0) return BD_OK;
+  std::vector<RepFlags> f((size_t)(r1 - r0), RepFlags{0, 0, kNoFailStep});
+  HIP_TRY(hipMemcpyAsync(s->flags + r0, f.data(), sizeof(RepFlags) * f.size(),
+                         hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return BD_OK;
+}
+
+// Drops both cached step graphs once the stream has drained, so that no
+// queued replay of either can outlive its executable graph.
+int drop_graphs(bd_sys* s) {
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (bd_sys::StepGraph* g : {&s->graph_plain, &s->graph_fused}) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    g->exec = nullptr;
+    g->steps = 0;
+  }
+  return BD_OK;
+}
+
+// The engine's per-call result buffer, at least `count` elements of T.
+template <class T>
+int call_buffer(bd_sys* s, size_t count, T** out) {
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  if (bytes > s->callbuf_bytes) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->callbuf) (void)hipFree(s->callbuf);
+    s->callbuf = nullptr;
+    s->callbuf_bytes = 0;
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(BD_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    s->callbuf = p;
+    s->callbuf_bytes = bytes;
+  }
+  *out = static_cast<T*>(s->callbuf);
   return BD_OK;
 }
 
@@ -3093,10 +3169,12 @@ int check_rep(const bd_sys* s, int r) {
 }
 
 void free_all(bd_sys* s) {
-  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  if (s->graph_plain.exec) (void)hipGraphExecDestroy(s->graph_plain.exec);
+  if (s->graph_fused.exec) (void)hipGraphExecDestroy(s->graph_fused.exec);
   if (s->tbl_brick) (void)hipFree(s->tbl_brick);
   if (s->tbl_split) (void)hipFree(s->tbl_split);
   if (s->origins) (void)hipFree(s->origins);
+  if (s->callbuf) (void)hipFree(s->callbuf);
   void* lists[] = {s->nlist, s->ncount, s->rbx, s->rby, s->rbz, s->lflag, s->list_ok};
   for (void* p : lists)
     if (p) (void)hipFree(p);
@@ -3440,7 +3518,10 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
   if (e == hipSuccess) e = hipMemset(s->cnt, 0, sizeof(int) * s->cells_total);
   if (e == hipSuccess) e = hipMemset(s->rs, 0, sizeof(int) * s->rs_total);
   if (e == hipSuccess) e = hipMemset(s->draws, 0, sizeof(uint64_t) * n_replicas);
-  if (e == hipSuccess) e = hipMemset(s->flags, 0, sizeof(int) * n_replicas);
+  if (e == hipSuccess) {
+    const std::vector<RepFlags> f0((size_t)n_replicas, RepFlags{0, 0, kNoFailStep});
+    e = hipMemcpy(s->flags, f0.data(), sizeof(RepFlags) * f0.size(), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess) e = hipMemcpy(s->step, &s->ni, sizeof(int64_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     k_rng_seed<<<(int)s->tbl_lane_h.size(), kBlk, 0, s->stream>>>(s->reps_d, s->tbl_lane,
@@ -3523,6 +3604,7 @@ int bd_set_positions(bd_sys* s, int32_t replica, const double* xyz) {
   rc = copy_in3(s, replica, xyz, s->px, s->py, s->pz);
   if (rc == BD_OK) rc = copy_in3(s, replica, xyz, s->rx, s->ry, s->rz);
   if (rc == BD_OK) rc = copy_in3(s, replica, xyz, s->ix, s->iy, s->iz);
+  if (rc == BD_OK) rc = clear_flags(s, replica, replica + 1);
   if (rc == BD_OK) {
     s->have_pos[replica] = true;
     s->located = false;
@@ -3698,13 +3780,19 @@ int bd_update(bd_sys* s) {
   return rc;
 }
 
-// Capture `steps` steps into one executable graph: whole steps, or with
+// Capture `steps` steps into an executable graph: whole steps, or with
 // `fused` the steady state of the fused update, `steps` pairs of (forces of
-// the current positions + the next move, binning).
+// the current positions + the next move, binning).  The graph of the other
+// kind stays cached.  A graph is replaced only after the stream has drained,
+// so no queued replay outlives its executable graph.
 static int build_graph(bd_sys* s, int steps, bool fused) {
-  if (s->graph) {
-    HIP_TRY(hipGraphExecDestroy(s->graph));
-    s->graph = nullptr;
+  bd_sys::StepGraph& sg = fused ? s->graph_fused : s->graph_plain;
+  if (sg.exec && sg.steps == steps) return BD_OK;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (sg.exec) {
+    HIP_TRY(hipGraphExecDestroy(sg.exec));
+    sg.exec = nullptr;
+    sg.steps = 0;
   }
   hipGraph_t g = nullptr;
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -3717,21 +3805,45 @@ static int build_graph(bd_sys* s, int steps, bool fused) {
     enqueue_steps(s, steps);
   }
   HIP_TRY(hipStreamEndCapture(s->stream, &g));
-  hipError_t e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+  hipError_t e = hipGraphInstantiate(&sg.exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    s->graph = nullptr;
+    sg.exec = nullptr;
     return fail(BD_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
-  s->graph_steps = steps;
-  s->graph_fused = fused;
+  sg.steps = steps;
   return BD_OK;
+}
+
+// How bd_step runs nsteps with the current settings: the fused path replays
+// (forces + move, binning) pairs G at a time when nsteps > G (the call's
+// first update and last forces run outside the graph); otherwise whole
+// steps are replayed G at a time and the remainder runs eagerly.
+static bool step_uses_fused(const bd_sys* s, int nsteps) {
+  return s->graph_mode > 0 && nsteps > s->graph_mode && fusable(s);
+}
+
+static int prepare_graphs(bd_sys* s, int nsteps) {
+  const int G = s->graph_mode;
+  if (s->persist > 0 || G <= 0) return BD_OK;
+  if (step_uses_fused(s, nsteps)) return build_graph(s, G, true);
+  if (nsteps >= G) return build_graph(s, G, false);
+  return BD_OK;
+}
+
+int bd_prepare_step(bd_sys* s, int32_t nsteps) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (nsteps < 0) return fail(BD_E_ARG, "nsteps must be >= 0");
+  return prepare_graphs(s, nsteps);
 }
 
 int bd_step(bd_sys* s, int32_t nsteps) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (nsteps < 0) return fail(BD_E_ARG, "nsteps must be >= 0");
   int rc = require_positions(s);
+  if (rc != BD_OK) return rc;
+  // the graphs are built first, so that a failure leaves no step half taken
+  rc = prepare_graphs(s, nsteps);
   if (rc != BD_OK) return rc;
   if (!s->located) {
     launch_bin(s);
@@ -3755,19 +3867,14 @@ int bd_step(bd_sys* s, int32_t nsteps) {
     return rc == BD_OK ? list_invalidate(s) : rc;
   }
   const int G = s->graph_mode;
-  if (G > 0 && nsteps > G && fusable(s)) {
+  if (step_uses_fused(s, nsteps)) {
     // the kernels of enqueue_steps(s, nsteps) in one run: the first step's
     // update, nsteps - 1 fused pairs (graph-replayed G at a time), the forces
     // of the final positions
-    // (the graph is built first, so that a failure leaves no step half taken)
-    if (!s->graph || s->graph_steps != G || !s->graph_fused) {
-      rc = build_graph(s, G, true);
-      if (rc != BD_OK) return rc;
-    }
     launch_update(s, 1);
     launch_cells(s, 1);
     int pairs = nsteps - 1;
-    for (; pairs >= G; pairs -= G) HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+    for (; pairs >= G; pairs -= G) HIP_TRY(hipGraphLaunch(s->graph_fused.exec, s->stream));
     for (; pairs > 0; pairs--) {
       launch_forces_move(s);
       launch_cells(s, 1);
@@ -3778,16 +3885,8 @@ int bd_step(bd_sys* s, int32_t nsteps) {
     return check_flags(s);
   }
   int left = nsteps;
-  if (G > 0 && left >= G) {
-    if (!s->graph || s->graph_steps != G || s->graph_fused) {
-      rc = build_graph(s, G, false);
-      if (rc != BD_OK) return rc;
-    }
-    while (left >= G) {
-      HIP_TRY(hipGraphLaunch(s->graph, s->stream));
-      left -= G;
-    }
-  }
+  if (G > 0)
+    for (; left >= G; left -= G) HIP_TRY(hipGraphLaunch(s->graph_plain.exec, s->stream));
   enqueue_steps(s, left);
   HIP_TRY(hipGetLastError());
   s->ni += nsteps;
@@ -3797,11 +3896,9 @@ int bd_step(bd_sys* s, int32_t nsteps) {
 int bd_set_graph_steps(bd_sys* s, int32_t steps) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (steps < 0 || steps > 256) return fail(BD_E_ARG, "graph steps must be 0..256");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  if (s->graph) {
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-    s->graph_steps = 0;
+  {
+    const int rc_ = drop_graphs(s);
+    if (rc_ != BD_OK) return rc_;
   }
   s->graph_mode = steps;
   return BD_OK;
@@ -3831,11 +3928,9 @@ int bd_get_graph_steps(const bd_sys* s, int32_t* steps) {
 
 int bd_set_brick(bd_sys* s, int32_t brick) {
   if (!s) return fail(BD_E_ARG, "null handle");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  if (s->graph) {
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-    s->graph_steps = 0;
+  {
+    const int rc_ = drop_graphs(s);
+    if (rc_ != BD_OK) return rc_;
   }
   const int rc = setup_bricks(s, brick);
   return rc == BD_OK ? list_invalidate(s) : rc;
@@ -3847,11 +3942,9 @@ int bd_set_force_order(bd_sys* s, int32_t order) {
     return fail(BD_E_ARG, "order must be BD_ORDER_REFERENCE, _SPLIT4 or _SPLIT8");
   if (order != BD_ORDER_REFERENCE && phs_kind(s->pot_kind))
     return fail(BD_E_ARG, "the 49-50 potentials run in the reference order only");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  if (s->graph) {
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-    s->graph_steps = 0;
+  {
+    const int rc_ = drop_graphs(s);
+    if (rc_ != BD_OK) return rc_;
   }
   if (order != BD_ORDER_REFERENCE) {
     for (int r = 0; r < s->R; r++)
@@ -3921,11 +4014,9 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
     launch_cells(s, 0);
     s->located = true;
   }
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  if (s->graph) {
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-    s->graph_steps = 0;
+  {
+    const int rc_ = drop_graphs(s);
+    if (rc_ != BD_OK) return rc_;
   }
   // kernel shapes: the row, slab and chunked slab walks at three deferral
   // depths, the LDS bricks.  The list walk is not among them: forces-only
@@ -4006,11 +4097,9 @@ int bd_set_walk(bd_sys* s, int32_t walk) {
   if (walk != BD_WALK_ROWS && walk != BD_WALK_SLABS && walk != BD_WALK_CHUNK2 &&
       walk != BD_WALK_CHUNK4 && walk != BD_WALK_LIST)
     return fail(BD_E_ARG, "walk must be BD_WALK_ROWS, _SLABS, _CHUNK2, _CHUNK4 or _LIST");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  if (s->graph) {
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-    s->graph_steps = 0;
+  {
+    const int rc_ = drop_graphs(s);
+    if (rc_ != BD_OK) return rc_;
   }
   if (walk == BD_WALK_LIST) {
     const int rc = list_setup(s);
@@ -4023,11 +4112,9 @@ int bd_set_walk(bd_sys* s, int32_t walk) {
 int bd_set_list_skin(bd_sys* s, double skin) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (!(skin > 0.0 && skin <= 2.0)) return fail(BD_E_ARG, "skin must lie in (0, 2]");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  if (s->graph) {
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-    s->graph_steps = 0;
+  {
+    const int rc_ = drop_graphs(s);
+    if (rc_ != BD_OK) return rc_;
   }
   s->skin = skin;
   return s->nlist ? list_setup(s) : BD_OK;
@@ -4048,11 +4135,9 @@ int bd_get_walk(const bd_sys* s, int32_t* walk) {
 int bd_set_pass_split(bd_sys* s, int32_t split) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (split < 0 || split > 2) return fail(BD_E_ARG, "pass split must be 0, 1 or 2");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  if (s->graph) {
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-    s->graph_steps = 0;
+  {
+    const int rc_ = drop_graphs(s);
+    if (rc_ != BD_OK) return rc_;
   }
   s->pass_split = split;
   return BD_OK;
@@ -4067,11 +4152,9 @@ int bd_get_pass_split(const bd_sys* s, int32_t* split) {
 int bd_set_fused_update(bd_sys* s, int32_t on) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (on < 0 || on > 2) return fail(BD_E_ARG, "fused update must be 0, 1 or 2");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  if (s->graph) {
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-    s->graph_steps = 0;
+  {
+    const int rc_ = drop_graphs(s);
+    if (rc_ != BD_OK) return rc_;
   }
   s->fuse = on;
   return BD_OK;
@@ -4088,11 +4171,9 @@ int bd_set_defer_depth(bd_sys* s, int32_t depth) {
   if (depth != 0 && depth != 16 && depth != 32) return fail(BD_E_ARG, "depth must be 0, 16 or 32");
   if (depth != 0 && !s->defer_ok)
     return fail(BD_E_ARG, "a grid has too many slots for deferred pair records");
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  if (s->graph) {
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-    s->graph_steps = 0;
+  {
+    const int rc_ = drop_graphs(s);
+    if (rc_ != BD_OK) return rc_;
   }
   s->defer = depth;
   return BD_OK;
@@ -4179,6 +4260,8 @@ int bd_checkpoint_load(bd_sys* s, const char* path) {
   if (rc != BD_OK) return rc;
   s->ni = h.step;
   HIP_TRY(hipMemcpy(s->step, &s->ni, sizeof(int64_t), hipMemcpyHostToDevice));
+  rc = clear_flags(s, 0, s->R);
+  if (rc != BD_OK) return rc;
   for (int r = 0; r < s->R; r++) s->have_pos[r] = true;
   s->located = false;  // cells are rebuilt (and the stored forces recomputed) on the next step
   return list_invalidate(s);
@@ -4352,7 +4435,7 @@ int bd_gder(bd_sys* s, int32_t replica, int32_t nbins, double range, uint64_t* c
     return fail(BD_E_ARG, "nbins must be in 1..512 and range > 0");
   const RepDev& R = s->reps[replica];
   unsigned long long* c_d = nullptr;
-  rc = dev_alloc(&c_d, (size_t)4 * nbins);
+  rc = call_buffer(s, (size_t)4 * nbins, &c_d);
   if (rc != BD_OK) return rc;
   hipError_t e = hipMemsetAsync(c_d, 0, sizeof(unsigned long long) * 4 * nbins, s->stream);
   const int iblocks = (R.n + kGderThreads - 1) / kGderThreads;
@@ -4371,7 +4454,6 @@ int bd_gder(bd_sys* s, int32_t replica, int32_t nbins, double range, uint64_t* c
     e = hipMemcpyAsync(h.data(), c_d, sizeof(unsigned long long) * 4 * nbins,
                        hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  (void)hipFree(c_d);
   if (e != hipSuccess) return fail(BD_E_HIP, std::string("bd_gder: ") + hipGetErrorString(e));
   for (size_t k = 0; k < h.size(); k++) counts[k] += h[k];
   return BD_OK;
@@ -4399,7 +4481,7 @@ int bd_fself_origins(bd_sys* s, double q, int32_t nslots, double* out) {
   if (!s->origins) return fail(BD_E_STATE, "no origin stored");
   double* o_d = nullptr;
   const size_t cnt = (size_t)s->R * nslots * 3;
-  int rc = dev_alloc(&o_d, cnt);
+  int rc = call_buffer(s, cnt, &o_d);
   if (rc != BD_OK) return rc;
   k_fself_origins<<<dim3(s->R, nslots), kOriginThreads, 0, s->stream>>>(
       s->reps_d, s->rx, s->ry, s->rz, s->origins, (size_t)3 * s->n_total, s->n_total, q, o_d);
@@ -4407,7 +4489,6 @@ int bd_fself_origins(bd_sys* s, double q, int32_t nslots, double* out) {
   if (e == hipSuccess)
     e = hipMemcpyAsync(out, o_d, cnt * sizeof(double), hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  (void)hipFree(o_d);
   if (e != hipSuccess) return fail(BD_E_HIP, std::string("bd_fself_origins: ") + hipGetErrorString(e));
   return BD_OK;
 }
@@ -4420,7 +4501,7 @@ int bd_gder_all(bd_sys* s, int32_t nbins, double range, uint64_t* counts) {
   for (const RepDev& R : s->reps) nmax = std::max(nmax, R.n);
   const size_t cnt = (size_t)s->R * 4 * nbins;
   unsigned long long* c_d = nullptr;
-  int rc = dev_alloc(&c_d, cnt);
+  int rc = call_buffer(s, cnt, &c_d);
   if (rc != BD_OK) return rc;
   hipError_t e = hipMemsetAsync(c_d, 0, sizeof(unsigned long long) * cnt, s->stream);
   const int iblocks = (nmax + kGderThreads - 1) / kGderThreads;
@@ -4436,7 +4517,6 @@ int bd_gder_all(bd_sys* s, int32_t nbins, double range, uint64_t* counts) {
     e = hipMemcpyAsync(counts, c_d, sizeof(unsigned long long) * cnt, hipMemcpyDeviceToHost,
                        s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  (void)hipFree(c_d);
   if (e != hipSuccess) return fail(BD_E_HIP, std::string("bd_gder_all: ") + hipGetErrorString(e));
   return BD_OK;
 }
@@ -4446,10 +4526,10 @@ int bd_observables(bd_sys* s, int32_t replica, double q, bd_obs* out) {
   if (rc != BD_OK) return rc;
   if (!out) return fail(BD_E_ARG, "null output");
   const RepDev& R = s->reps[replica];
-  int* w_d = nullptr;
+  // six sums, then the replica index in the eighth word
   double* o_d = nullptr;
-  rc = dev_alloc(&w_d, 1);
-  if (rc == BD_OK) rc = dev_alloc(&o_d, 6);
+  rc = call_buffer(s, 8, &o_d);
+  int* w_d = rc == BD_OK ? reinterpret_cast<int*>(o_d + 7) : nullptr;
   hipError_t e = hipSuccess;
   double h[6] = {0, 0, 0, 0, 0, 0};
   if (rc == BD_OK) {
@@ -4464,8 +4544,6 @@ int bd_observables(bd_sys* s, int32_t replica, double q, bd_obs* out) {
       e = hipMemcpyAsync(h, o_d, sizeof(double) * 6, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   }
-  if (w_d) (void)hipFree(w_d);
-  if (o_d) (void)hipFree(o_d);
   if (rc != BD_OK) return rc;
   if (e != hipSuccess) return fail(BD_E_HIP, std::string("bd_observables: ") + hipGetErrorString(e));
   const bd_params& p = s->params[replica];

@@ -189,6 +189,10 @@ class Engine:
     def step(self, nsteps=1):
         check(self.lib.bd_step(self.h, int(nsteps)))
 
+    def prepare_step(self, nsteps):
+        """Build the graphs step(nsteps) would replay, without stepping."""
+        check(self.lib.bd_prepare_step(self.h, int(nsteps)))
+
     def autotune(self, trial_launches=10, cache_path=None):
         """Pick the fastest force-kernel shape (results are bit-identical)."""
         v = C.c_int32()

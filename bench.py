@@ -74,6 +74,27 @@ def pick_shape(eng, shape, trials):
             "chosen_by": "autotuner" if shape is None else "--shape"}
 
 
+def timed_launch_mode(eng, steps, graph_steps):
+    """Graphs of G = min(graph_steps, steps - 1) steps, built before the timed
+    region: a timed call of any length >= 2 replays at least one graph.
+    Returns the launch mode as the timed call runs it."""
+    G = max(1, min(graph_steps, steps - 1)) if graph_steps > 0 else 0
+    eng.set_graph_steps(G)
+    eng.prepare_step(steps)
+    fused = bool(eng.fused_update())
+    if G == 0:
+        return {"launch": "eager", "graph_steps": 0, "fused_update": fused,
+                "graph_replays": 0, "eager_steps": steps}
+    if fused and steps > G:
+        replays, eager = (steps - 1) // G, (steps - 1) % G
+        note = "first update and last forces eager; (forces + move, binning) pairs in graphs"
+    else:
+        replays, eager = steps // G, steps % G
+        note = "whole steps in graphs"
+    return {"launch": "graph" if replays > 0 else "eager", "graph_steps": G,
+            "fused_update": fused, "graph_replays": replays, "eager_steps": eager, "note": note}
+
+
 def run_plain(steps, warmup, graph_steps=16, shape=None):
     nb = ns = 16384
     side = box_side(nb, ns, 1.4, 1.0, 0.58)
@@ -89,7 +110,8 @@ def run_plain(steps, warmup, graph_steps=16, shape=None):
     shape = pick_shape(eng, shape, 20)
     log(f"[bench] force kernel {shape}")
     eng.mark_origin()
-    log(f"[bench] timed {steps} steps at T=0.1")
+    mode = timed_launch_mode(eng, steps, graph_steps)
+    log(f"[bench] timed {steps} steps at T=0.1, {mode}")
     t0 = time.perf_counter()
     eng.step(steps)
     dt = time.perf_counter() - t0
@@ -117,6 +139,7 @@ def run_plain(steps, warmup, graph_steps=16, shape=None):
     eng.close()
     return {"N": nb + ns, "side": side, "seconds": dt, "rate": (nb + ns) * steps / dt,
             "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p, "shape": shape,
+            "mode": mode,
             "gder_s": gder_s, "gder_hits": gder_hits, "fused_ms": fused_ms, "cv3d_s": cv3d_s}
 
 
@@ -140,7 +163,8 @@ def run_batch(steps, warmup, graph_steps=16, shape=None):
     shape = pick_shape(eng, shape, 10)
     log(f"[bench] batch force kernel {shape}")
     eng.mark_origin()
-    log(f"[bench] batch timed {steps} steps")
+    mode = timed_launch_mode(eng, steps, graph_steps)
+    log(f"[bench] batch timed {steps} steps, {mode}")
     t0 = time.perf_counter()
     eng.step(steps)
     dt = time.perf_counter() - t0
@@ -149,7 +173,8 @@ def run_batch(steps, warmup, graph_steps=16, shape=None):
     eng.close()
     tot = int(n.sum())
     return {"replicas": len(n), "sum_N": tot, "seconds": dt, "rate": tot * steps / dt,
-            "kernel_ms": ms.tolist(), "shape": shape, "specs": specs, "T": T, "x": xs}
+            "kernel_ms": ms.tolist(), "shape": shape, "specs": specs, "T": T, "x": xs,
+            "mode": mode}
 
 
 def run_stress(steps, warmup, graph_steps=16, shape=None):
@@ -164,6 +189,7 @@ def run_stress(steps, warmup, graph_steps=16, shape=None):
     eng.set_positions(x)
     eng.step(warmup)
     shape = pick_shape(eng, shape, 10)
+    timed_launch_mode(eng, steps, graph_steps)
     t0 = time.perf_counter()
     eng.step(steps)
     dt = time.perf_counter() - t0
@@ -285,10 +311,10 @@ def main():
                     help="force kernel as BRICK,DEPTH[,WALK[,SPLIT]] (e.g. 0,16,3,2) instead of "
                          "autotuning")
     ap.add_argument("--graph-steps", type=int, default=256,
-                    help="steps per captured HIP graph (the engine's default is 16; the "
-                         "timed runs here are long enough for 256, which leaves fewer gaps "
-                         "between graph launches); 0 launches every kernel eagerly (same "
-                         "kernels, for kernel tracing)")
+                    help="largest number of steps per captured HIP graph (the engine's "
+                         "default is 16); a timed call of K steps uses graphs of "
+                         "min(this, K - 1) steps, built before the timed region; 0 launches "
+                         "every kernel eagerly (same kernels, for kernel tracing)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the runs, copy a 1 GiB buffer 3 times (a plain copy kernel of "
                          "known traffic, the scale for FETCH_SIZE / WRITE_SIZE counters)")
@@ -350,7 +376,7 @@ def main():
         "config": {"workload": "aging quench N=32768 phi=0.58 WCA sigma 1.4/1.0, dt=1e-4, "
                                "T 1.0 -> 0.1", "N": N, "side": round(plain["side"], 4),
                    "potential": "wca", "cells": "23^3 big + 32^3 small, ntags 8",
-                   "force_kernel": plain["shape"]},
+                   "force_kernel": plain["shape"], "launch_mode": plain["mode"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic, "traffic_unit": "B per launch (rocprofv3)",
@@ -383,7 +409,7 @@ def main():
                         "forces_hbm_GB_s": B_FORCES * batch["sum_N"] / (batch["kernel_ms"][2] * 1e-3) / 1e9,
                         "forces_traffic": TRAFFIC_FORCES_BATCH,
                         "cpu_baseline": cpu_batch,
-                        "force_kernel": batch["shape"]}
+                        "force_kernel": batch["shape"], "launch_mode": batch["mode"]}
     print(json.dumps(out), flush=True)
     if a.copy_calibration:
         copy_calibration()

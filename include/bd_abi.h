@@ -157,16 +157,30 @@ int bd_neighbours(bd_sys* s, int32_t replica, int32_t maxn, int32_t* counts, int
  * protocol temperature; advances the rannew64 lanes and the step index. */
 int bd_update(bd_sys* s);
 /* nsteps full steps: update -> temperature -> cell locate -> forces
- * (main.cu:494-586); replayed through a captured HIP graph. */
+ * (main.cu:494-586); replayed through captured HIP graphs.
+ * Errors: cell overflow (BD_E_CELL_OVERFLOW) and non-finite positions
+ * (BD_E_NAN) are raised on the device and read once, when the call ends; the
+ * message names the replica and the step index of the first update (or
+ * binning) that raised it, and the whole call's later steps ran on that
+ * state.  A replica's error stays raised (every later call fails) until its
+ * state is replaced by bd_set_positions or bd_checkpoint_load. */
 int bd_step(bd_sys* s, int32_t nsteps);
+/* Builds (and caches) the graphs a bd_step(s, nsteps) call with the current
+ * settings would replay, without running any step (no reference
+ * counterpart): lets a caller keep graph instantiation out of a timed or
+ * latency-critical call. */
+int bd_prepare_step(bd_sys* s, int32_t nsteps);
 
 /* Launch mode of bd_step (no reference counterpart): steps per captured
- * HIP graph, 1..256 (default 16), or 0 to enqueue every kernel eagerly (for
- * tracing tools that do not follow graph launches).  A call shorter than a
- * graph runs eagerly.  With the fused update the graph holds that many
- * (forces + next move, binning) pairs, and the call's first update and last
- * forces are launched once around the replays.  Results are bit-identical
- * in every mode. */
+ * HIP graph G, 1..256 (default 16), or 0 to enqueue every kernel eagerly (for
+ * tracing tools that do not follow graph launches).  Without the fused
+ * update a call replays floor(nsteps / G) graphs of G whole steps and runs
+ * the remainder eagerly.  With the fused update a call of more than G steps
+ * runs its first update and last forces eagerly and replays the nsteps - 1
+ * (forces + next move, binning) pairs G at a time, the remainder eagerly; a
+ * call of at most G steps uses the whole-step path.  The two graphs are
+ * cached side by side, so alternating call lengths rebuild neither.
+ * Results are bit-identical in every mode. */
 int bd_set_graph_steps(bd_sys* s, int32_t steps);
 int bd_get_graph_steps(const bd_sys* s, int32_t* steps);
 

@@ -613,6 +613,60 @@ def test_cell_overflow_is_reported():
     e.close()
 
 
+def test_error_names_first_failing_step_and_clears():
+    """A cell overflow in the middle of a graph-replayed run is reported with
+    the step index of the update that caused it (the oracle, stepped one
+    step at a time, fails at the same step), and the record clears once the
+    replica's positions are replaced."""
+    p = make_params(200, 200, phi=0.05, ntags_big=3, ntags_sml=3, T0=300.0, Tf=300.0, seed=52)
+    x = np.random.default_rng(12).random((400, 3)) * p.side
+    o = orc.System(p)
+    o.set_positions(x)
+    assert o.cell_locate() == 0
+    o.compute_forces()
+    first = None
+    for _ in range(60):
+        ni = o.step_index()
+        rc = o.step(1)
+        if rc:
+            first = (ni, rc)
+            break
+    o.close()
+    assert first is not None and first[0] > 16, first  # inside the second graph replay
+    e = Engine(p)
+    e.set_positions(x)
+    with pytest.raises(abi.BDError) as err:
+        e.step(60)
+    assert err.value.code == first[1]
+    assert f"at step {first[0]}:" in str(err.value), str(err.value)
+    with pytest.raises(abi.BDError):  # the record stays until the state is replaced
+        e.cell_locate()
+    e.set_positions(x)
+    e.cell_locate()
+    e.close()
+
+
+def test_alternating_call_lengths_keep_both_graphs(config1_positions):
+    """Calls of alternating lengths (fused pairs in graphs, whole-step graphs,
+    eager remainders) against one eager run: the same trajectory."""
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, T0=1.0, Tf=0.2, t0=0.0, tf=0.0, seed=23)
+    runs = []
+    for calls in ((17, 16, 40, 3, 16, 33), (125,)):
+        e = Engine(p)
+        e.set_walk(abi.BD_WALK_CHUNK4)
+        e.set_pass_split(2)
+        assert e.fused_update() == 1
+        e.set_graph_steps(16 if len(calls) > 1 else 0)
+        e.set_positions(x)
+        for c in calls:
+            e.step(c)
+        runs.append([e.positions(), e.raw_positions(), *e.forces(), e.step_index()])
+        e.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
 def test_relaxed_configuration_has_no_overlaps(config1_positions):
     side, x = config1_positions
     d = x[:, None, :] - x[None, :, :]

@@ -343,6 +343,106 @@ def test_update_zero_temperature_closed_form():
     assert np.array_equal(s.raw_positions(), x + d)
 
 
+def jittered_lattice(nb, ns, phi, seed, jitter=0.08, **kw):
+    """nb + ns particles (big first) on random sites of a cubic lattice filling
+    the box at packing fraction phi, each moved by up to `jitter` lattice
+    spacings: pairs inside the cutoffs, none deeply overlapping."""
+    p = make_params(nb, ns, phi=phi, seed=seed, **kw)
+    n = nb + ns
+    m = int(math.ceil(n ** (1.0 / 3.0)))
+    a = p.side / m
+    rng = np.random.default_rng(seed)
+    sites = rng.permutation(m ** 3)[:n]
+    ijk = np.stack([sites % m, (sites // m) % m, sites // (m * m)], axis=1)
+    x = (ijk + 0.5 + rng.uniform(-jitter, jitter, (n, 3))) * a
+    return p, np.mod(x, p.side)
+
+
+def np_wca_pairs(p, x):
+    """All ordered pairs i != j inside the cutoff with numpy: separation
+    r_ij = r_i - r_j (minimum image), distance, U and dU/dr from the textbook
+    WCA forms (not from the oracle's potential routine):
+      WCA:          U = 4 (s^12 - s^6) + 1,   r < 2^(1/6) sigma_ij
+      WCA modified: U = s^12 - 2 s^6 + 1,     r < sigma_ij
+    with s = sigma_ij / r and sigma_ij = (sigma_i + sigma_j) / 2."""
+    n = p.n_big + p.n_sml
+    sig = np.where(np.arange(n) < p.n_big, p.sigma_big, p.sigma_sml)
+    sij = 0.5 * (sig[:, None] + sig[None, :])
+    d = x[:, None, :] - x[None, :, :]
+    d -= p.side * np.round(d / p.side)
+    r = np.sqrt((d ** 2).sum(-1))
+    np.fill_diagonal(r, np.inf)
+    if p.potential == BD_POT_WCA:
+        rc = 2.0 ** (1.0 / 6.0) * sij
+    else:
+        rc = sij
+    inside = r < rc
+    s = np.where(inside, sij / np.where(inside, r, 1.0), 0.0)
+    if p.potential == BD_POT_WCA:
+        U = np.where(inside, 4.0 * (s ** 12 - s ** 6) + 1.0, 0.0)
+        dU = np.where(inside, -(48.0 * s ** 12 - 24.0 * s ** 6) / np.where(inside, r, 1.0), 0.0)
+    else:
+        U = np.where(inside, s ** 12 - 2.0 * s ** 6 + 1.0, 0.0)
+        dU = np.where(inside, -(12.0 * s ** 12 - 12.0 * s ** 6) / np.where(inside, r, 1.0), 0.0)
+    return r, U, dU, inside
+
+
+@pytest.mark.parametrize("pot", [BD_POT_WCA, BD_POT_WCA_MODIFIED])
+@pytest.mark.parametrize("phi", [0.30, 0.55])
+def test_energy_virial_z_pinned_independently(pot, phi):
+    """main.cu:627-638 (e_s = sum over species s of pot / 2 / N_s, pot the sum of
+    U over a particle's partners) and main.cu:801-806 with get_forces.cu:134,265
+    (Z = 1 + sum_i vir_i / (3 N_tot T), vir_i = -1/2 sum_j (r_j - r_i).F_ij,
+    i.e. sum_i vir_i = sum over unordered pairs of r_ij.F_ij = -r dU/dr) from
+    an all-pairs numpy evaluation, on 80 big + 120 small particles (N_big,
+    N_sml and N_tot all differ) at T = 0.7."""
+    nb, ns, T = 80, 120, 0.7
+    p, x = jittered_lattice(nb, ns, phi, 31, potential=pot, T0=T, Tf=T)
+    s = orc.System(p)
+    s.set_positions(x)
+    assert s.cell_locate() == 0
+    s.compute_forces()
+    o = s.observables(7.0)
+    s.close()
+    r, U, dU, inside = np_wca_pairs(p, x)
+    assert inside.sum() >= 50  # the check has pairs to see
+    e_i = U.sum(axis=1)  # pot of particle i: sum over its partners
+    e_b = 0.5 * e_i[:nb].sum() / nb
+    e_s = 0.5 * e_i[nb:].sum() / ns
+    w = -np.where(inside, r, 0.0) * dU  # r_ij . F_ij per ordered pair
+    vir = 0.5 * w.sum()  # each unordered pair once
+    z = 1.0 + vir / (3.0 * (nb + ns) * T)
+    assert vir > 0  # repulsive pairs push outwards
+    rel = lambda a, b: abs(a - b) / abs(b)  # noqa: E731
+    assert rel(o["epot_big"], e_b) <= 1e-12
+    assert rel(o["epot_sml"], e_s) <= 1e-12
+    assert rel(o["virial"], vir) <= 1e-12
+    assert rel(o["z"] - 1.0, z - 1.0) <= 1e-12
+    assert o["temperature"] == T
+    # each plausible slip moves a value far outside the tolerance: a
+    # missing 1/2, a flipped virial, N_big for N_tot, the species swapped
+    for wrong_e in (2 * e_b, e_i[:nb].sum() * 0.5 / ns):
+        assert rel(o["epot_big"], wrong_e) > 1e-3
+    for wrong_z in (1.0 - vir / (3.0 * (nb + ns) * T), 1.0 + vir / (3.0 * nb * T),
+                    1.0 + 2 * vir / (3.0 * (nb + ns) * T), 1.0 + vir / (3.0 * (nb + ns))):
+        assert rel(o["z"] - 1.0, wrong_z - 1.0) > 1e-3
+
+
+def test_z_is_one_without_interactions():
+    # no pair within any cutoff: every virial is zero and Z = 1 exactly
+    p, x = jittered_lattice(40, 60, 0.01, 5, jitter=0.0, T0=0.4, Tf=0.4)
+    r, _, _, inside = np_wca_pairs(p, x)
+    assert not inside.any()
+    s = orc.System(p)
+    s.set_positions(x)
+    assert s.cell_locate() == 0
+    s.compute_forces()
+    o = s.observables(7.0)
+    s.close()
+    assert o["virial"] == 0.0 and o["z"] == 1.0
+    assert o["epot_big"] == 0.0 and o["epot_sml"] == 0.0
+
+
 def test_free_diffusion_msd_and_fself():
     # zero forces: MSD = 6 D t and F_s(k, t) = exp(-k^2 D t), D = T / sigma
     p, x = random_dilute(1500, 1500, 0.05, 8, T0=0.7, Tf=0.7, dt=1e-3)
