@@ -107,6 +107,19 @@ __device__ inline void raise_flag(RepFlags* f, int rep, int bit, int64_t ni) {
   atomicMin(&f[rep].first_step, (long long)ni);
 }
 
+// Id-ordered copy of a particle's position with its cell in its own
+// species' grid, written by the binning for the neighbour-list walk (one
+// 32-byte gather per listed candidate).
+struct __align__(32) PosCell {
+  double x, y, z;
+  unsigned int cell;  // x | y << 10 | z << 20
+  unsigned int pad_;
+};
+
+__device__ inline unsigned int pack_cell(int cx, int cy, int cz) {
+  return (unsigned int)cx | ((unsigned int)cy << 10) | ((unsigned int)cz << 20);
+}
+
 thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
@@ -421,7 +434,8 @@ __device__ __forceinline__ void k_cells_block(
             const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
             const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
             RepFlags* __restrict__ flags, int64_t* __restrict__ step,
-            uint64_t* __restrict__ draws, int advance, int* __restrict__ lflag = nullptr) {
+            uint64_t* __restrict__ draws, int advance, int* __restrict__ lflag = nullptr,
+            PosCell* __restrict__ pos4 = nullptr) {
   const int4 t = tbl[vb];
   if (advance && threadIdx.x == 0) {
     if (vb == 0) *step = *step + 1;
@@ -495,7 +509,9 @@ __device__ __forceinline__ void k_cells_block(
 #pragma unroll
             for (int q = 0; q < 8; q++) rank += id[q] < id[k] ? 1 : 0;
             const int p = R.p_off + sp_off + id[k];
-            cp[start + rank] = make_double4(px[p], py[p], pz[p], (double)(sp_off + id[k]));
+            const double qx = px[p], qy = py[p], qz = pz[p];
+            cp[start + rank] = make_double4(qx, qy, qz, (double)(sp_off + id[k]));
+            if (pos4) pos4[p] = PosCell{qx, qy, qz, pack_cell(row % n, row / n, z), 0u};
           }
         }
       } else {
@@ -504,7 +520,9 @@ __device__ __forceinline__ void k_cells_block(
           int rank = 0;
           for (int q = 0; q < m; q++) rank += slots[q] < idk ? 1 : 0;
           const int p = R.p_off + sp_off + idk;
-          cp[start + rank] = make_double4(px[p], py[p], pz[p], (double)(sp_off + idk));
+          const double qx = px[p], qy = py[p], qz = pz[p];
+          cp[start + rank] = make_double4(qx, qy, qz, (double)(sp_off + idk));
+          if (pos4) pos4[p] = PosCell{qx, qy, qz, pack_cell(row % n, row / n, z), 0u};
         }
       }
       cnt[R.cell_off[g] + c] = occ;
@@ -523,9 +541,10 @@ __global__ void __launch_bounds__(kBlk)
             const double* __restrict__ pz, int* __restrict__ nocup, int* __restrict__ cnt,
             const int* __restrict__ ins, int* __restrict__ rs, double4* __restrict__ cpos,
             RepFlags* __restrict__ flags, int64_t* __restrict__ step,
-            uint64_t* __restrict__ draws, int advance, int* __restrict__ lflag) {
+            uint64_t* __restrict__ draws, int advance, int* __restrict__ lflag,
+            PosCell* __restrict__ pos4) {
   k_cells_block((int)blockIdx.x, reps, tbl, px, py, pz, nocup, cnt, ins, rs, cpos, flags, step,
-                draws, advance, lflag);
+                draws, advance, lflag, pos4);
 }
 
 // Distance along one axis from a point to the cell at offset d of its home
@@ -1735,30 +1754,41 @@ __global__ void __launch_bounds__(4 * kBlk)
 
 // ---------------------------------------------------------------------------
 // Neighbour-list walk (BD_WALK_LIST, no reference counterpart).  Each
-// particle keeps the ids of every particle of either grid within rc + skin,
-// rebuilt for the whole replica on the first force evaluation after a move
-// took some particle more than half the skin from its position at the last
-// build (Verlet's criterion).  A step then tests only the listed candidates.
-// The reference order of the pair terms is recovered exactly: a partner
-// inside the cutoff gets the key (pass, offset of its cell from the
-// particle's cell in the walk's stencil order, id), which is its position in
-// the cell walk (cells in (di, dj, dk) order, ids ascending within a cell),
-// and the partners are evaluated in key order after an insertion sort in
-// LDS.  For a pair inside the cutoff the minimum image is the image of the
-// cell walk (grids of at least 2 nrange + 3 cells), so the separations and
-// the terms are the walk's to the bit.  A particle whose list or sorted
-// partners would overflow walks its cells instead.
+// particle keeps the ids of every particle of either species within its
+// pass cutoff plus a skin.  A move that takes some particle of a replica
+// more than half the skin from its position at the replica's last build
+// raises the replica's rebuild flag (Verlet's criterion: while no particle
+// has moved that far, every pair inside the cutoff is listed).  In the step
+// whose binning follows that move, the replica's forces are computed by the
+// cell walk while k_list_build rebuilds its lists from the same cells, on a
+// second stream beside the force kernel; every other replica tests only its
+// listed candidates, gathered by id from pos4, the copy of the positions
+// that k_cells writes with each particle's cell in its own species' grid.
+// The reference order of the pair terms
+// is recovered exactly: a partner inside the cutoff gets the key (pass,
+// offset of its cell from the particle's cell in the stencil's (di, dj, dk)
+// order, id), its place in the cell walk (cells in (di, dj, dk) order, ids
+// ascending within a cell, get_forces.cu:360-416,495-550); the partners are
+// insertion-sorted by key in LDS and their terms added in that order.  The
+// pass's grid is the partner's own species' grid in both passes, so pos4
+// carries the partner's cell there.  For a pair
+// inside the cutoff the compare-based minimum image equals the reference's
+// dx - L floor(dx/L + 0.5) (the cutoff lies below L/2), so separations and
+// terms are the cell walk's to the bit.  A particle whose list or sorted
+// partners overflow walks its cells instead.
 // ---------------------------------------------------------------------------
-constexpr int kListCap = 48;  // listed candidates per particle
-constexpr int kHitCap = 32;   // partners inside the cutoff sorted per particle
+constexpr int kListCap = 48;   // listed candidates per particle
+constexpr int kHitCap = 32;    // partners inside the cutoff sorted per particle
+constexpr int kListChunk = 4;  // list entries whose positions are gathered together
 
 struct ListArgs {
-  unsigned int* nlist;   // [kListCap][n_total]: replica-local id | pass bit
-  int* ncount;           // listed candidates per particle, -1: walk the cells
+  unsigned int* nlist;      // [kListCap][n_total]: replica-local ids
+  int* ncount;              // listed candidates per particle, -1: walk the cells
   double *rbx, *rby, *rbz;  // raw positions at the build
   const double *rx, *ry, *rz;
-  const int* lflag;      // rebuild flags, [2 R]
-  const int* list_ok;    // [R]: the grids hold the list stencil
+  const int* lflag;         // rebuild flags, [2 R]
+  const int* list_ok;       // [R]: the grids hold the list stencil
+  const PosCell* pos4;      // by particle, written by k_cells
   int n_total;
   double skin;
 };
@@ -1767,60 +1797,182 @@ __device__ inline double min_image(double e, double L, double halfL) {
   return e > halfL ? -L : (e < -halfL ? L : 0.0);
 }
 
-// Candidate list of replica-local particle i (global index p) at (x, y, z):
-// every particle of either grid closer than rc + skin, from the packed rows.
-// Returns the count, or -1 when it exceeds kListCap.
-__device__ inline int build_list(const RepDev& R, int i, int p, double x, double y, double z,
-                                 const int* __restrict__ rs, const double4* __restrict__ cpos,
-                                 const ListArgs& la) {
-  const int g = i < R.nb ? 0 : 1;
-  const double L = R.side, halfL = 0.5 * R.side;
-  int cnt = 0;
-  for (int pass = 0; pass < 2; pass++) {
-    const int gg = pass == 0 ? g : 1 - g;
-    const double rl = sqrt(pass == 0 ? R.rc2_same[g] : R.rc2_bs) + la.skin;
-    const double rl2 = rl * rl;
-    const int n = R.ncell[gg];
-    const int nr = (int)ceil(rl / R.cell_side[gg]);
-    int h[3];
-    if (!cell_coords(R, gg, x, y, z, h[0], h[1], h[2])) return -1;
-    const int* rs_g = rs + R.rs_off[gg];
-    const double4* cp = cpos + R.tag_off[gg];
-    const double self = pass == 0 ? (double)i : -1.0;
-    const unsigned int pb = pass == 0 ? 0u : kPassBit;
-    for (int di = -nr; di <= nr; di++) {
-      int a = h[0] + di;
-      a = a < 0 ? a + n : (a >= n ? a - n : a);
-      for (int dj = -nr; dj <= nr; dj++) {
-        int b = h[1] + dj;
-        b = b < 0 ? b + n : (b >= n ? b - n : b);
-        const int* rrow = rs_g + (a + n * b) * (n + 1);
-        for (int dk = -nr; dk <= nr; dk++) {
-          int c = h[2] + dk;
-          c = c < 0 ? c + n : (c >= n ? c - n : c);
-          const int s1 = rrow[c + 1];
-          for (int sl = rrow[c]; sl < s1; sl++) {
-            const double4 q = cp[sl];
-            if (q.w == self) continue;
-            double dx = q.x - x, dy = q.y - y, dz = q.z - z;
-            dx += min_image(dx, L, halfL);
-            dy += min_image(dy, L, halfL);
-            dz += min_image(dz, L, halfL);
-            if (dx * dx + dy * dy + dz * dz < rl2) {
-              if (cnt < kListCap) la.nlist[(size_t)cnt * la.n_total + p] = (unsigned int)q.w | pb;
-              cnt++;
-            }
-          }
+// List build, kBuildLanes lanes per particle: lane `sub` walks the rows
+// (di, dj) of both passes' stencils whose running index is congruent to sub,
+// appending each particle of the pass's grid closer than the pass cutoff
+// plus the skin to the particle's list through an LDS counter (the list
+// order is free: the force kernel sorts the partners it finds).  The stencil
+// reaches up to 2 cells each way on grids of at least 2 reach + 3 cells (so
+// cell images are minimum images); rows and cells whose nearest point lies
+// beyond the list radius are skipped, and a row's candidates are loaded
+// kListChunk at a time.
+constexpr int kBuildLanes = 8;
+constexpr int kBuildThreads = kBlk * kBuildLanes;
+
+struct BuildPass {
+  int h[3];        // home cell
+  int n, nr;       // cells per side, stencil reach
+  double cs, rl2;  // cell side, squared list radius
+  double ux, uy, uz;
+  double g2z[5];   // squared z gaps of dk = -2 .. 2
+  double selfw;    // replica-local index skipped (-1: none)
+  const int* rs;
+  const double4* cp;
+};
+
+__device__ inline bool build_pass(const RepDev& R, int g, int pass, int i, double x, double y,
+                                  double z, double skin, const int* __restrict__ rs,
+                                  const double4* __restrict__ cpos, BuildPass& b) {
+  const int gg = pass == 0 ? g : 1 - g;
+  const double rl = sqrt(pass == 0 ? R.rc2_same[g] : R.rc2_bs) + skin;
+  // a margin over (rc + skin)^2: a pair exactly that far at the build stays
+  // listed whatever the rounding
+  b.rl2 = rl * rl * (1.0 + 1e-12);
+  b.cs = R.cell_side[gg];
+  b.n = R.ncell[gg];
+  b.nr = (int)ceil(rl / b.cs);
+  if (!cell_coords(R, gg, x, y, z, b.h[0], b.h[1], b.h[2])) return false;
+  b.ux = x - (double)b.h[0] * b.cs;
+  b.uy = y - (double)b.h[1] * b.cs;
+  b.uz = z - (double)b.h[2] * b.cs;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const double gz = slab_gap(k - 2, b.cs, b.uz);
+    b.g2z[k] = gz * gz;
+  }
+  b.selfw = pass == 0 ? (double)i : -1.0;
+  b.rs = rs + R.rs_off[gg];
+  b.cp = cpos + R.tag_off[gg];
+  return true;
+}
+
+// Row r (0 .. (2 nr + 1)^2 - 1, di major) of a pass: its candidates within
+// the list radius appended to the list column `out` (stride) through *ctr.
+__device__ inline void build_row(const RepDev& R, const BuildPass& b, int r, double x, double y,
+                                 double z, unsigned int* __restrict__ out, size_t stride,
+                                 int* ctr) {
+  const double L = R.side;
+  const double lim = b.rl2 + 1e-9;
+  const int w = 2 * b.nr + 1;
+  const int di = r / w - b.nr, dj = r % w - b.nr;
+  const double gx = slab_gap(di, b.cs, b.ux), gy = slab_gap(dj, b.cs, b.uy);
+  const double gxy2 = gx * gx + gy * gy;
+  if (gxy2 > lim) return;
+  const int n = b.n;
+  const double shx = image_shift(b.h[0] + di, n, L), shy = image_shift(b.h[1] + dj, n, L);
+  int a = b.h[0] + di, c = b.h[1] + dj;
+  a = a < 0 ? a + n : (a >= n ? a - n : a);
+  c = c < 0 ? c + n : (c >= n ? c - n : c);
+  int klo = 0, khi = 0;
+  for (int k = 1; k <= b.nr; k++) {
+    if (gxy2 + b.g2z[2 - k] <= lim) klo = -k;
+    if (gxy2 + b.g2z[2 + k] <= lim) khi = k;
+  }
+  const int* rrow = b.rs + (a + n * c) * (n + 1);
+  const int z0 = b.h[2] + klo, z1 = b.h[2] + khi;
+  const bool wlo = z0 < 0, whi = z1 >= n;
+  int lo[2], hi[2];
+  double shz[2];
+  lo[0] = rrow[wlo ? z0 + n : z0];
+  hi[0] = rrow[(wlo || whi) ? n : z1 + 1];
+  shz[0] = wlo ? -L : 0.0;
+  lo[1] = hi[1] = 0;
+  shz[1] = whi ? L : 0.0;
+  if (wlo || whi) {
+    lo[1] = rrow[0];
+    hi[1] = rrow[wlo ? z1 + 1 : z1 - n + 1];
+  }
+  for (int part = 0; part < 2; part++) {
+    for (int s0 = lo[part]; s0 < hi[part]; s0 += kListChunk) {
+      double4 q[kListChunk];
+#pragma unroll
+      for (int u = 0; u < kListChunk; u++) q[u] = b.cp[s0 + u < hi[part] ? s0 + u : s0];
+#pragma unroll
+      for (int u = 0; u < kListChunk; u++) {
+        const double dx = (q[u].x - x) + shx;
+        const double dy = (q[u].y - y) + shy;
+        const double dz = (q[u].z - z) + shz[part];
+        if (s0 + u < hi[part] && dx * dx + dy * dy + dz * dz < b.rl2 && q[u].w != b.selfw) {
+          const int k = atomicAdd(ctr, 1);
+          if (k < kListCap) out[(size_t)k * stride] = (unsigned int)q[u].w;
         }
       }
     }
   }
-  return cnt <= kListCap ? cnt : -1;
 }
 
-// Forces of every particle from its list.  The candidates' positions are
-// read from px/py/pz by id, so the update cannot be fused into this kernel
-// (a moved particle would be seen by lanes still testing it).
+// Rebuilds the lists of the replicas whose rebuild flag is raised.  The
+// grid strides over the particle block table: each thread checks one entry,
+// the flagged entries are compacted in LDS, then the workgroup builds them
+// one after the other (kBuildLanes lanes per particle) and records the build
+// positions.  A step without rebuilds costs the checks only.
+__global__ void __launch_bounds__(kBuildThreads)
+    k_list_build(const RepDev* __restrict__ reps, const int4* __restrict__ tbl, int ntbl,
+                 const double* __restrict__ px, const double* __restrict__ py,
+                 const double* __restrict__ pz, const int* __restrict__ rs,
+                 const double4* __restrict__ cpos, const uint64_t* __restrict__ draws,
+                 ListArgs la) {
+  __shared__ int todo[kBuildThreads];
+  __shared__ int ntodo;
+  __shared__ int lcnt[kBlk];
+  if (threadIdx.x == 0) ntodo = 0;
+  __syncthreads();
+  {
+    const int e = (int)blockIdx.x + (int)threadIdx.x * (int)gridDim.x;
+    if (e < ntbl) {
+      const int4 t = tbl[e];
+      if (t.x >= 0 && la.list_ok[t.x] &&
+          la.lflag[2 * t.x + list_parity(reps[t.x], draws[t.x], 0)])
+        todo[atomicAdd(&ntodo, 1)] = e;
+    }
+  }
+  __syncthreads();
+  const int nt = ntodo;
+  const int pl = (int)threadIdx.x / kBuildLanes, sub = (int)threadIdx.x % kBuildLanes;
+  for (int k = 0; k < nt; k++) {
+    const int4 t = tbl[todo[k]];
+    const RepDev& R = reps[t.x];
+    if (threadIdx.x < kBlk) lcnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = t.y + pl;
+    const bool live = i < t.z;
+    const int p = R.p_off + i;
+    if (live) {
+      const int g = i < R.nb ? 0 : 1;
+      const double x = px[p], y = py[p], z = pz[p];
+      BuildPass b0, b1;
+      if (build_pass(R, g, 0, i, x, y, z, la.skin, rs, cpos, b0) &&
+          build_pass(R, g, 1, i, x, y, z, la.skin, rs, cpos, b1)) {
+        const int w0 = (2 * b0.nr + 1) * (2 * b0.nr + 1);
+        const int w1 = (2 * b1.nr + 1) * (2 * b1.nr + 1);
+        for (int r = sub; r < w0 + w1; r += kBuildLanes) {
+          if (r < w0)
+            build_row(R, b0, r, x, y, z, la.nlist + p, (size_t)la.n_total, &lcnt[pl]);
+          else
+            build_row(R, b1, r - w0, x, y, z, la.nlist + p, (size_t)la.n_total, &lcnt[pl]);
+        }
+      } else if (sub == 0) {
+        atomicAdd(&lcnt[pl], kListCap + 1);  // no cell: the particle walks its cells
+      }
+    }
+    __syncthreads();
+    if (live && sub == 0) {
+      const int c = lcnt[pl];
+      la.ncount[p] = c <= kListCap ? c : -1;
+      la.rbx[p] = la.rx[p];
+      la.rby[p] = la.ry[p];
+      la.rbz[p] = la.rz[p];
+    }
+    __syncthreads();
+  }
+}
+
+// Forces of every particle from its list; replicas whose lists are being
+// rebuilt in this step walk their cells (the row walk, no LDS).  The listed
+// candidates' positions are gathered kListChunk at a time, the partners
+// inside the cutoff are insertion-sorted by their cell-walk key, then their
+// terms are added in key order (pair_add's operations on the same
+// separations), the partners' positions again gathered kListChunk at a time.
 template <int POT>
 __global__ void __launch_bounds__(kBlk)
     k_forces_list(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
@@ -1840,76 +1992,85 @@ __global__ void __launch_bounds__(kBlk)
   const double x = px[p], y = py[p], z = pz[p];
   const double L = R.side, halfL = 0.5 * R.side;
   double fx = 0.0, fy = 0.0, fz = 0.0, v = 0.0, u = 0.0;
-  bool walk = !R.fast || !la.list_ok[t.x];
+  bool walk = !R.fast || !la.list_ok[t.x] ||
+              la.lflag[2 * t.x + list_parity(R, draws[t.x], 0)] != 0;
   if (!walk) {
-    int cnt;
-    if (la.lflag[2 * t.x + list_parity(R, draws[t.x], 0)]) {
-      cnt = build_list(R, i, p, x, y, z, rs, cpos, la);
-      la.ncount[p] = cnt;
-      la.rbx[p] = la.rx[p];
-      la.rby[p] = la.ry[p];
-      la.rbz[p] = la.rz[p];
-    } else {
-      cnt = la.ncount[p];
-    }
-    int h[2][3];
-    if (cnt < 0 || !cell_coords(R, g, x, y, z, h[0][0], h[0][1], h[0][2]) ||
-        !cell_coords(R, 1 - g, x, y, z, h[1][0], h[1][1], h[1][2]))
-      walk = true;
+    const int cnt = la.ncount[p];
+    int ho[2][3];  // the particle's cell in its own grid (pass 0) and the other (pass 1)
+    walk = cnt < 0 || !cell_coords(R, g, x, y, z, ho[0][0], ho[0][1], ho[0][2]) ||
+           !cell_coords(R, 1 - g, x, y, z, ho[1][0], ho[1][1], ho[1][2]);
+    const int nr0 = R.nrange_same[g], nr1 = R.nrange_cross[g];
+    const double rc2s = R.rc2_same[g], rc2x = R.rc2_bs;
+    const PosCell* pc = la.pos4 + R.p_off;
     unsigned int* kc = keys + threadIdx.x;
     int nh = 0;
-    for (int c = 0; c < cnt && !walk; c++) {
-      const unsigned int e = la.nlist[(size_t)c * la.n_total + p];
-      const int pass = (int)(e >> 31);
-      const int j = (int)(e & ~kPassBit);
-      const int q = R.p_off + j;
-      const double qx = px[q], qy = py[q], qz = pz[q];
-      const double ex = qx - x, ey = qy - y, ez = qz - z;
-      const double dx = ex + min_image(ex, L, halfL);
-      const double dy = ey + min_image(ey, L, halfL);
-      const double dz = ez + min_image(ez, L, halfL);
-      const double dist2 = dx * dx + dy * dy + dz * dz;
-      if (dist2 < (pass ? R.rc2_bs : R.rc2_same[g])) {
-        // the partner's place in the cell walk: stencil cell, then id
-        const int gg = pass ? 1 - g : g;
-        const int nr = pass ? R.nrange_cross[g] : R.nrange_same[g];
-        const int n = R.ncell[gg];
-        int cj[3] = {0, 0, 0};
-        cell_coords(R, gg, qx, qy, qz, cj[0], cj[1], cj[2]);
-        int d[3];
+    for (int c0 = 0; c0 < cnt && !walk; c0 += kListChunk) {
+      unsigned int id[kListChunk];
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-          const int o = cj[a] - h[pass][a];
-          d[a] = (o > nr ? o - n : (o < -nr ? o + n : o)) + nr;
-        }
-        const unsigned int key = ((unsigned int)pass << 31) |
-                                 ((unsigned int)((d[0] * (2 * nr + 1) + d[1]) * (2 * nr + 1) +
-                                                 d[2]) << 24) |
-                                 (unsigned int)j;
-        if (nh == kHitCap) {
-          walk = true;
-          break;
-        }
-        int pos = nh;
-        while (pos > 0 && kc[(pos - 1) * kBlk] > key) {
-          kc[pos * kBlk] = kc[(pos - 1) * kBlk];
-          pos--;
-        }
-        kc[pos * kBlk] = key;
-        nh++;
-      }
-    }
-    if (!walk) {
-      for (int e = 0; e < nh; e++) {
-        const unsigned int key = kc[e * kBlk];
-        const int pass = (int)(key >> 31);
-        const int q = R.p_off + (int)(key & 0xffffffu);
-        const double ex = px[q] - x, ey = py[q] - y, ez = pz[q] - z;
+      for (int k = 0; k < kListChunk; k++)
+        id[k] = la.nlist[(size_t)(c0 + k < cnt ? c0 + k : c0) * la.n_total + p];
+      PosCell q[kListChunk];
+#pragma unroll
+      for (int k = 0; k < kListChunk; k++) q[k] = pc[id[k]];
+#pragma unroll
+      for (int k = 0; k < kListChunk; k++) {
+        if (c0 + k >= cnt) break;
+        const bool cross = ((int)id[k] < R.nb) != (g == 0);
+        const double ex = q[k].x - x, ey = q[k].y - y, ez = q[k].z - z;
         const double dx = ex + min_image(ex, L, halfL);
         const double dy = ey + min_image(ey, L, halfL);
         const double dz = ez + min_image(ez, L, halfL);
-        const double dist2 = dx * dx + dy * dy + dz * dz;
-        pair_add<POT>(R, pass ? R.sigma_bs : R.sigma[g], dist2, dx, dy, dz, fx, fy, fz, v, u);
+        if (dx * dx + dy * dy + dz * dz < (cross ? rc2x : rc2s)) {
+          // the partner's cell in its own grid, the pass's grid
+          const int gj = cross ? 1 - g : g;
+          const unsigned int cw = q[k].cell;
+          const int cj[3] = {(int)(cw & 1023u), (int)((cw >> 10) & 1023u),
+                             (int)((cw >> 20) & 1023u)};
+          const int nr = cross ? nr1 : nr0;
+          const int n = R.ncell[gj];
+          const int* h = ho[cross ? 1 : 0];
+          int d[3];
+#pragma unroll
+          for (int a = 0; a < 3; a++) {
+            const int o = cj[a] - h[a];
+            d[a] = (o > nr ? o - n : (o < -nr ? o + n : o)) + nr;
+          }
+          const int w = 2 * nr + 1;
+          const unsigned int key = (cross ? kPassBit : 0u) |
+                                   ((unsigned int)((d[0] * w + d[1]) * w + d[2]) << 24) | id[k];
+          if (nh == kHitCap) {
+            walk = true;
+            break;
+          }
+          int pos = nh;
+          while (pos > 0 && kc[(pos - 1) * kBlk] > key) {
+            kc[pos * kBlk] = kc[(pos - 1) * kBlk];
+            pos--;
+          }
+          kc[pos * kBlk] = key;
+          nh++;
+        }
+      }
+    }
+    if (!walk) {
+      for (int e0 = 0; e0 < nh; e0 += kListChunk) {
+        unsigned int key[kListChunk];
+        PosCell q[kListChunk];
+#pragma unroll
+        for (int k = 0; k < kListChunk; k++) key[k] = kc[(e0 + k < nh ? e0 + k : e0) * kBlk];
+#pragma unroll
+        for (int k = 0; k < kListChunk; k++) q[k] = pc[key[k] & 0xffffffu];
+#pragma unroll
+        for (int k = 0; k < kListChunk; k++) {
+          if (e0 + k >= nh) break;
+          const double ex = q[k].x - x, ey = q[k].y - y, ez = q[k].z - z;
+          const double dx = ex + min_image(ex, L, halfL);
+          const double dy = ey + min_image(ey, L, halfL);
+          const double dz = ez + min_image(ez, L, halfL);
+          const double dist2 = dx * dx + dy * dy + dz * dz;
+          pair_add<POT>(R, (key[k] & kPassBit) ? R.sigma_bs : R.sigma[g], dist2, dx, dy, dz, fx,
+                        fy, fz, v, u);
+        }
       }
     }
   }
@@ -2622,6 +2783,10 @@ struct bd_sys {
   int* ncount = nullptr;
   double *rbx = nullptr, *rby = nullptr, *rbz = nullptr;
   int* lflag = nullptr;    // [2 R] rebuild flags
+  // the list rebuild runs on a second stream beside the force kernel
+  hipStream_t stream2 = nullptr;
+  PosCell* pos4 = nullptr;  // positions with own-grid cells, by particle
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int* list_ok = nullptr;  // [R]
   bool defer_ok = true;  // every grid's slot index fits the 26-bit record
   std::vector<int4> tbl_split_h;
@@ -2648,7 +2813,7 @@ void launch(bd_sys* s, void (*k)(P...), int nblk, int nthr, A... a) {
   std::apply(
       [&](auto&... e) {
         void* ptrs[] = {static_cast<void*>(&e)...};
-        if (s->tev[0])
+        if (s->tev[0] || s->tev[1])
           (void)hipExtLaunchKernel(reinterpret_cast<const void*>(k), dim3(nblk), dim3(nthr), ptrs,
                                    0, s->stream, s->tev[0], s->tev[1], 0);
         else
@@ -2686,7 +2851,7 @@ void launch_cells(bd_sys* s, int advance) {
   const int nb = (int)s->tbl_cell_h.size();
   launch(s, k_cells, nb, kBlk, s->reps_d, s->tbl_cell, s->px, s->py, s->pz, s->nocup, s->cnt,
          s->ins, s->rs, s->cpos, s->flags, s->step, s->draws, advance,
-         list_on(s) ? s->lflag : nullptr);
+         list_on(s) ? s->lflag : nullptr, list_on(s) ? s->pos4 : nullptr);
 }
 
 MoveArgs move_args(bd_sys* s) {
@@ -2694,9 +2859,32 @@ MoveArgs move_args(bd_sys* s) {
                   s->ry,   s->rz,    s->nocup, s->ins,   s->flags,       list_move(s)};
 }
 
+// The list walk's forces: k_list_build over the flagged replicas on the
+// second stream, beside the force kernel on the engine's stream (forked
+// from and joined back into it, also under graph capture).  Under kernel
+// timing the two run one after the other and the event pair spans both.
 void launch_list(bd_sys* s, const int4* tbl, int nblk) {
-  const ListArgs la{s->nlist, s->ncount, s->rbx, s->rby,   s->rbz,        s->rx,
-                    s->ry,    s->rz,     s->lflag, s->list_ok, s->n_total, s->skin};
+  const ListArgs la{s->nlist, s->ncount, s->rbx,     s->rby,      s->rbz,
+                    s->rx,    s->ry,     s->rz,      s->lflag,    s->list_ok,
+                    s->pos4,  s->n_total, s->skin};
+  const int bgrid = std::max(std::min(nblk, 1024), (nblk + kBuildThreads - 1) / kBuildThreads);
+  const hipEvent_t e0 = s->tev[0], e1 = s->tev[1];
+  const bool timed = e0 || e1;
+  if (timed) {
+    s->tev[1] = nullptr;  // the build stamps the start
+    launch(s, k_list_build, bgrid, kBuildThreads, s->reps_d, tbl, nblk, s->px, s->py, s->pz,
+           s->rs, s->cpos, s->draws, la);
+    s->tev[0] = nullptr;  // the force kernel the end
+    s->tev[1] = e1;
+  } else {
+    (void)hipEventRecord(s->ev_fork, s->stream);
+    (void)hipStreamWaitEvent(s->stream2, s->ev_fork, 0);
+    hipStream_t own = s->stream;
+    s->stream = s->stream2;
+    launch(s, k_list_build, bgrid, kBuildThreads, s->reps_d, tbl, nblk, s->px, s->py, s->pz,
+           s->rs, s->cpos, s->draws, la);
+    s->stream = own;
+  }
   switch (s->pot_kind) {
     case BD_POT_WCA:
       launch(s, k_forces_list<BD_POT_WCA>, nblk, kBlk, s->reps_d, tbl, s->px, s->py, s->pz, s->rs,
@@ -2711,6 +2899,12 @@ void launch_list(bd_sys* s, const int4* tbl, int nblk) {
              s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->draws, la);
       break;
   }
+  if (!timed) {
+    (void)hipEventRecord(s->ev_join, s->stream2);
+    (void)hipStreamWaitEvent(s->stream, s->ev_join, 0);
+  }
+  s->tev[0] = e0;
+  s->tev[1] = e1;
 }
 
 // Every list is stale: the next force evaluation of each replica rebuilds.
@@ -2735,7 +2929,11 @@ int list_setup(bd_sys* s) {
     if (rc == BD_OK) rc = dev_alloc(&s->rbz, (size_t)s->n_total);
     if (rc == BD_OK) rc = dev_alloc(&s->lflag, 2 * (size_t)s->R);
     if (rc == BD_OK) rc = dev_alloc(&s->list_ok, (size_t)s->R);
+    if (rc == BD_OK) rc = dev_alloc(&s->pos4, (size_t)s->n_total);
     if (rc != BD_OK) return rc;
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
   }
   std::vector<int> ok(s->R, 0);
   for (int r = 0; r < s->R; r++) {
@@ -2746,12 +2944,24 @@ int list_setup(bd_sys* s) {
         const int gg = pass == 0 ? g : 1 - g;
         const double rl = std::sqrt(pass == 0 ? R.rc2_same[g] : R.rc2_bs) + s->skin;
         const int nr = (int)std::ceil(rl / R.cell_side[gg]);
-        good = 2 * nr + 1 <= R.ncell[gg] && rl < 0.5 * R.side;
+        // the build's stencil reaches nr <= 2 cells and sees minimum images
+        good = nr <= 2 && 2 * nr + 3 <= R.ncell[gg] && rl < 0.5 * R.side;
       }
     ok[r] = good ? 1 : 0;
   }
   HIP_TRY(hipMemcpy(s->list_ok, ok.data(), sizeof(int) * ok.size(), hipMemcpyHostToDevice));
   return list_invalidate(s);
+}
+
+// With the list walk on and cells located, bins the current positions again
+// so that k_cells rewrites pos4 (same cells: the positions did not move).
+int list_refresh(bd_sys* s) {
+  if (!list_on(s) || !s->located) return BD_OK;
+  launch_bin(s);
+  launch_cells(s, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return BD_OK;
 }
 
 // UPD: the force kernel also performs the next step's update (see k_forces).
@@ -3175,7 +3385,7 @@ void free_all(bd_sys* s) {
   if (s->tbl_split) (void)hipFree(s->tbl_split);
   if (s->origins) (void)hipFree(s->origins);
   if (s->callbuf) (void)hipFree(s->callbuf);
-  void* lists[] = {s->nlist, s->ncount, s->rbx, s->rby, s->rbz, s->lflag, s->list_ok};
+  void* lists[] = {s->nlist, s->ncount, s->rbx, s->rby, s->rbz, s->lflag, s->list_ok, s->pos4};
   for (void* p : lists)
     if (p) (void)hipFree(p);
   void* ptrs[] = {s->tbl_part, s->tbl_lane, s->tbl_cell, s->reps_d, s->px,   s->py,
@@ -3185,6 +3395,9 @@ void free_all(bd_sys* s) {
                   s->lcg,      s->draws,    s->step,     s->flags};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+  if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+  if (s->stream2) (void)hipStreamDestroy(s->stream2);
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -3863,7 +4076,8 @@ int bd_step(bd_sys* s, int32_t nsteps) {
     HIP_TRY(hipGetLastError());
     s->ni += nsteps;
     rc = check_flags(s);
-    // the persistent kernel keeps no neighbour lists
+    // the persistent kernel keeps no neighbour lists and writes no pos4
+    if (rc == BD_OK) rc = list_refresh(s);
     return rc == BD_OK ? list_invalidate(s) : rc;
   }
   const int G = s->graph_mode;
@@ -3932,7 +4146,8 @@ int bd_set_brick(bd_sys* s, int32_t brick) {
     const int rc_ = drop_graphs(s);
     if (rc_ != BD_OK) return rc_;
   }
-  const int rc = setup_bricks(s, brick);
+  int rc = setup_bricks(s, brick);
+  if (rc == BD_OK) rc = list_refresh(s);
   return rc == BD_OK ? list_invalidate(s) : rc;
 }
 
@@ -3961,7 +4176,8 @@ int bd_set_force_order(bd_sys* s, int32_t order) {
     s->tbl_split_h = tbl;
   }
   s->order = order;
-  return list_invalidate(s);
+  const int rc = list_refresh(s);
+  return rc == BD_OK ? list_invalidate(s) : rc;
 }
 
 // Autotuner: average duration of the force kernel for every brick shape the
@@ -4106,7 +4322,8 @@ int bd_set_walk(bd_sys* s, int32_t walk) {
     if (rc != BD_OK) return rc;
   }
   s->walk = walk;
-  return list_invalidate(s);
+  const int rc = list_refresh(s);
+  return rc == BD_OK ? list_invalidate(s) : rc;
 }
 
 int bd_set_list_skin(bd_sys* s, double skin) {

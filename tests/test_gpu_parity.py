@@ -293,6 +293,42 @@ def test_list_walk_bit_identical(config1_positions):
     e.close()
 
 
+def test_list_walk_batch_bit_identical():
+    """The list walk on an uneven batch (replicas rebuild at their own steps:
+    T from 0.05 to 1.5), switched on after the cells are located and back and
+    forth during the run, against the default slab walk: every replica's
+    state equal after 300 steps of graph replays and eager steps."""
+    specs = [(300, 340, 0.52, 1.5, 1), (700, 650, 0.55, 0.2, 2), (1000, 1000, 0.60, 0.05, 3),
+             (520, 480, 0.58, 0.7, 4)]
+    params, starts = [], []
+    for nb, ns, phi, T, seed in specs:
+        side = box_side(nb, ns, 1.4, 1.0, phi)
+        starts.append(relax_overlaps(nb, ns, side, seed=seed))
+        params.append(make_params(nb, ns, side=side, T0=1.0, Tf=T, t0=0.0, tf=0.0,
+                                  first_step=-5, seed=3000 + seed))
+    runs = []
+    for mode in ("slabs", "list"):
+        e = Engine(params)
+        for r, x in enumerate(starts):
+            e.set_positions(x, r)
+        e.cell_locate()
+        if mode == "list":
+            e.set_walk(abi.BD_WALK_LIST)  # after the cells exist
+        e.step(100)
+        if mode == "list":
+            e.set_walk(abi.BD_WALK_SLABS)
+            e.step(37)
+            e.set_walk(abi.BD_WALK_LIST)
+            e.step(163)
+        else:
+            e.step(200)
+        runs.append([[e.positions(r), e.raw_positions(r), *e.forces(r)] for r in range(len(specs))])
+        e.close()
+    for r in range(len(specs)):
+        for a, b in zip(runs[0][r], runs[1][r]):
+            assert np.array_equal(a, b), r
+
+
 @pytest.fixture(scope="module")
 def plain_positions():
     """N=32768 (16384 + 16384) at phi=0.58, no overlaps: the bench workload."""
