@@ -128,6 +128,7 @@ SIGNATURES = {
     "bd_gder": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, _U]),
     "bd_observables": (C.c_int, [_P, C.c_int32, C.c_double, C.POINTER(BDObs)]),
     "bd_gder_all": (C.c_int, [_P, C.c_int32, C.c_double, _U]),
+    "bd_gder_noalloc": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, _U]),
     "bd_set_origin": (C.c_int, [_P, C.c_int32]),
     "bd_fself_origins": (C.c_int, [_P, C.c_double, C.c_int32, _D]),
     "bd_cv3d_create": (C.c_int, [C.c_int32, C.c_double, C.POINTER(C.c_void_p)]),

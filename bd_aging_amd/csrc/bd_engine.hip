@@ -43,6 +43,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -54,6 +55,13 @@
 #include "bd_internal.h"
 
 #pragma clang fp contract(off)
+
+// Fixed histogram of bd_gder_noalloc: the four partial histograms of up to
+// 512 bins (the bd_gder limit), 16 KiB per GPU, held until the process ends.
+// The host-side name is only a stand-in: host code uses the per-device
+// address from hipGetSymbolAddress (gder_scratch_ptr).
+constexpr int kScratchWords = 4 * 512;
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
 
 namespace {
 
@@ -2724,6 +2732,7 @@ int dev_alloc(T** p, size_t count) {
 
 struct bd_sys {
   int R = 0;
+  int device = 0;  // the GPU current at bd_create
   std::vector<bd_params> params;
   std::vector<bd_geometry> geom;
   std::vector<RepDev> reps;
@@ -3372,6 +3381,27 @@ int call_buffer(bd_sys* s, size_t count, T** out) {
   return BD_OK;
 }
 
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current (bd_create).  Never the array's host-side name.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int gder_scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(BD_E_ARG, "device index beyond 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return BD_OK;
+  void* ptr = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[dev] = ptr;
+  return BD_OK;
+}
+
+void* gder_scratch_ptr(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return (dev >= 0 && dev < kMaxDevices) ? g_scratch_ptr[dev] : nullptr;
+}
+
 int check_rep(const bd_sys* s, int r) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (r < 0 || r >= s->R) return fail(BD_E_ARG, "replica index out of range");
@@ -3710,7 +3740,14 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
     delete s;
     return rc;
   }
-  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&s->device);
+  if (e == hipSuccess && gder_scratch_setup(s->device) != BD_OK) {
+    const std::string m = g_err;
+    free_all(s);
+    delete s;
+    return fail(BD_E_HIP, "bd_create: " + m);
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e == hipSuccess)
     e = hipMemcpy(s->tbl_part, s->tbl_part_h.data(), sizeof(int4) * s->tbl_part_h.size(),
                   hipMemcpyHostToDevice);
@@ -4673,6 +4710,40 @@ int bd_gder(bd_sys* s, int32_t replica, int32_t nbins, double range, uint64_t* c
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) return fail(BD_E_HIP, std::string("bd_gder: ") + hipGetErrorString(e));
   for (size_t k = 0; k < h.size(); k++) counts[k] += h[k];
+  return BD_OK;
+}
+
+// bd_gder without any allocation: the histogram is the fixed device array
+// `scratch` of the engine's GPU (through its looked-up address), loaded with
+// the caller's counts, added to by the same kernel and copied back.  Calls
+// that overlap on one GPU share that array and race.
+int bd_gder_noalloc(bd_sys* s, int32_t replica, int32_t nbins, double range, uint64_t* counts) {
+  int rc = check_rep(s, replica);
+  if (rc != BD_OK) return rc;
+  if (nbins < 1 || nbins > kGderMaxBins || 4 * nbins > kScratchWords || !(range > 0.0) ||
+      !counts)
+    return fail(BD_E_ARG, "nbins must be in 1..512 (the fixed histogram holds 4 x 512 bins) "
+                          "and range > 0");
+  unsigned long long* c_d = static_cast<unsigned long long*>(gder_scratch_ptr(s->device));
+  if (!c_d) return fail(BD_E_STATE, "no fixed histogram on this device");
+  const RepDev& R = s->reps[replica];
+  const size_t bytes = sizeof(unsigned long long) * 4 * (size_t)nbins;
+  hipError_t e = hipMemcpyAsync(c_d, counts, bytes, hipMemcpyHostToDevice, s->stream);
+  const int iblocks = (R.n + kGderThreads - 1) / kGderThreads;
+  int jsplit = std::max(1, std::min(64, 4096 / std::max(1, iblocks)));
+  const int j_chunk = ((R.n + jsplit - 1) / jsplit + kGderThreads - 1) / kGderThreads * kGderThreads;
+  jsplit = (R.n + j_chunk - 1) / j_chunk;
+  const size_t shmem = sizeof(unsigned int) * (kGderThreads / 64) * 4 * nbins;
+  if (e == hipSuccess) {
+    k_gder<<<dim3(iblocks, jsplit), kGderThreads, shmem, s->stream>>>(
+        s->reps_d, replica, s->px, s->py, s->pz, nbins, range, range / (double)nbins, j_chunk,
+        c_d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(counts, c_d, bytes, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess)
+    return fail(BD_E_HIP, std::string("bd_gder_noalloc: ") + hipGetErrorString(e));
   return BD_OK;
 }
 

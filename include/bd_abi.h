@@ -348,6 +348,14 @@ int bd_observables(bd_sys* s, int32_t replica, double q, bd_obs* out);
 /* The same histograms for every replica in one launch:
  * counts[r * 4 * nbins + ...] (overwritten, not added). */
 int bd_gder_all(bd_sys* s, int32_t nbins, double range, uint64_t* counts);
+/* bd_gder without any device allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory, such as code under HIP
+ * graph capture, tight serving loops or harnesses that forbid hipMalloc.
+ * The histogram is a fixed 16 KiB device array per GPU (4 x 512 bins),
+ * loaded with counts[], added to and copied back: the same result as
+ * bd_gder.  nbins above 512 is BD_E_ARG.  Every call on a GPU shares that
+ * array, so calls that overlap on one GPU (several threads or handles) race. */
+int bd_gder_noalloc(bd_sys* s, int32_t replica, int32_t nbins, double range, uint64_t* counts);
 
 /* Waiting-time origins (no reference counterpart: the reference re-runs the
  * binary per decade, run-fself.sh:149-197).  bd_set_origin stores the
