@@ -147,6 +147,7 @@ SIGNATURES = {
     "bd_mlp_set_params": (C.c_int, [_P, _F]),
     "bd_mlp_forward": (C.c_int, [_P, _F, C.c_int32, _F]),
     "bd_mlp_gradients": (C.c_int, [_P, _F, _F, C.c_int32, _F, _F]),
+    "bd_mlp_trace": (C.c_int, [_P, C.c_int32, C.c_int32, _F]),
     "bd_mlp_train_step": (C.c_int, [_P, _F, _F, C.c_int32, C.c_float, _F]),
     "bd_mlp_set_data": (C.c_int, [_P, _F, _F, C.c_int32]),
     "bd_mlp_train": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _F]),

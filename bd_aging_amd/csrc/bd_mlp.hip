@@ -34,7 +34,7 @@ struct bd_mlp {
   unsigned short *xb = nullptr, *h1b = nullptr, *h2b = nullptr, *dyb = nullptr,
                  *dz2b = nullptr, *dz1b = nullptr;
   float *x = nullptr, *t = nullptr, *z1 = nullptr, *z2 = nullptr, *y = nullptr, *dy = nullptr,
-        *dh = nullptr, *dz = nullptr, *loss = nullptr;
+        *dh = nullptr, *dz = nullptr, *dz1 = nullptr, *loss = nullptr;
   // device-resident data set
   float *dx = nullptr, *dt = nullptr;
   int n_data = 0;
@@ -233,7 +233,7 @@ void gemm(bd_mlp* s, const unsigned short* A, long sAm, long sAi, long sAk,
 
 int ensure_capacity(bd_mlp* s, int batch) {
   if (batch <= s->cap) return BD_OK;
-  float** fb[] = {&s->x, &s->z1, &s->z2, &s->y, &s->dy, &s->dh, &s->dz, &s->t};
+  float** fb[] = {&s->x, &s->z1, &s->z2, &s->y, &s->dy, &s->dh, &s->dz, &s->dz1, &s->t};
   unsigned short** hb[] = {&s->xb, &s->h1b, &s->h2b, &s->dyb, &s->dz2b, &s->dz1b};
   for (float** q : fb)
     if (*q) (void)hipFree(*q), *q = nullptr;
@@ -249,6 +249,7 @@ int ensure_capacity(bd_mlp* s, int batch) {
   MLP_TRY(hipMalloc((void**)&s->dy, M * B * s->n_out * sizeof(float)));
   MLP_TRY(hipMalloc((void**)&s->dh, M * B * wmax * sizeof(float)));
   MLP_TRY(hipMalloc((void**)&s->dz, M * B * wmax * sizeof(float)));
+  MLP_TRY(hipMalloc((void**)&s->dz1, M * B * s->hidden * sizeof(float)));
   MLP_TRY(hipMalloc((void**)&s->xb, M * B * s->n_in * sizeof(unsigned short)));
   MLP_TRY(hipMalloc((void**)&s->h1b, M * B * s->hidden * sizeof(unsigned short)));
   MLP_TRY(hipMalloc((void**)&s->h2b, M * B * s->hidden * sizeof(unsigned short)));
@@ -281,7 +282,9 @@ void forward(bd_mlp* s, int batch) {
 }
 
 // Backward of the mean-squared error against s->t (per member targets);
-// writes the gradients of every member into s->g.
+// writes the gradients of every member into s->g.  The FP32 gradients of the
+// pre-activations stay in s->dz (layer 2) and s->dz1 (layer 1), their BF16
+// copies in s->dz2b and s->dz1b (read back by bd_mlp_trace).
 // The members' losses go to loss_out[0..members).
 void backward(bd_mlp* s, int batch, float* loss_out) {
   const int M = s->members, H = s->hidden, I = s->n_in, O = s->n_out;
@@ -305,18 +308,20 @@ void backward(bd_mlp* s, int batch, float* loss_out) {
   gemm(s, s->dz2b, (long)batch * H, H, 1, s->wb + s->off_w2, P, 1, H, s->dh, (long)batch * H, H,
        1, batch, H, H);
   k_gelu_bwd<<<dim3(ew_blocks((size_t)batch * H), M), kEw, 0, s->stream>>>(s->dh, s->z1, batch,
-                                                                           H, s->dz, s->dz1b);
+                                                                           H, s->dz1, s->dz1b);
   gemm(s, s->xb, (long)batch * I, 1, I, s->dz1b, (long)batch * H, H, 1, s->g + s->off_w1, P, H,
        1, I, H, batch);
-  k_colsum<<<dim3((H + kEw - 1) / kEw, M), kEw, 0, s->stream>>>(s->dz, batch, H, s->g, s->off_b1,
-                                                                s->nparam);
+  k_colsum<<<dim3((H + kEw - 1) / kEw, M), kEw, 0, s->stream>>>(s->dz1, batch, H, s->g,
+                                                                s->off_b1, s->nparam);
 }
 
 void adam(bd_mlp* s, float lr) {
   s->adam_t++;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float c1 = 1.0f - std::pow(b1, (float)s->adam_t);
-  const float c2 = 1.0f - std::pow(b2, (float)s->adam_t);
+  // bias corrections in FP64: 1 - 0.999^t in FP32 would lose ~1e-5 to
+  // cancellation in the first steps
+  const float c1 = (float)(1.0 - std::pow(0.9, (double)s->adam_t));
+  const float c2 = (float)(1.0 - std::pow(0.999, (double)s->adam_t));
   const size_t n = (size_t)s->members * s->nparam;
   k_adam<<<ew_blocks(n), kEw, 0, s->stream>>>(s->p, s->g, s->m1, s->m2, s->wb, n, lr, b1, b2,
                                                eps, c1, c2);
@@ -395,8 +400,8 @@ int bd_mlp_destroy(bd_mlp* s) {
   if (!s) return BD_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   void* ptrs[] = {s->p,  s->g,   s->m1,  s->m2,  s->wb,   s->loss, s->x,    s->t,  s->z1,
-                  s->z2, s->y,   s->dy,  s->dh,  s->dz,   s->xb,   s->h1b,  s->h2b, s->dyb,
-                  s->dz2b, s->dz1b, s->dx, s->dt, s->idx, s->lhist};
+                  s->z2, s->y,   s->dy,  s->dh,  s->dz,   s->dz1,  s->xb,   s->h1b, s->h2b,
+                  s->dyb, s->dz2b, s->dz1b, s->dx, s->dt, s->idx, s->lhist};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -469,6 +474,49 @@ int bd_mlp_gradients(bd_mlp* s, const float* x, const float* t, int32_t batch, f
                            hipMemcpyDeviceToHost, s->stream));
   MLP_TRY(hipStreamSynchronize(s->stream));
   return BD_OK;
+}
+
+// The intermediates of member `member` from the last bd_mlp_gradients (or
+// bd_mlp_train_step) call on `batch` rows, as floats, one array after the
+// other: z1, h1, z2, h2 [batch][hidden]; y, dy, dyb [batch][n_out]; dz2,
+// dz2b, dz1, dz1b [batch][hidden] (h*, dyb, dz*b: the BF16 operands of the
+// matrix cores; z*, dy, dz*: FP32).
+int bd_mlp_trace(bd_mlp* s, int32_t member, int32_t batch, float* out) {
+  if (!s || !out || member < 0 || member >= s->members || batch < 1 || batch > s->cap)
+    return bdi::set_error(BD_E_ARG, "bad arguments");
+  const size_t bh = (size_t)batch * s->hidden, bo = (size_t)batch * s->n_out;
+  const size_t mh = (size_t)member * bh, mo = (size_t)member * bo;
+  std::vector<unsigned short> u(bh);
+  auto f32 = [&](const float* src, size_t n, float* dst) -> int {
+    MLP_TRY(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    return BD_OK;
+  };
+  auto b16 = [&](const unsigned short* src, size_t n, float* dst) -> int {
+    MLP_TRY(hipMemcpyAsync(u.data(), src, n * sizeof(unsigned short), hipMemcpyDeviceToHost,
+                           s->stream));
+    MLP_TRY(hipStreamSynchronize(s->stream));
+    for (size_t k = 0; k < n; k++) {
+      const unsigned int w = (unsigned int)u[k] << 16;
+      std::memcpy(&dst[k], &w, 4);
+    }
+    return BD_OK;
+  };
+  float* o = out;
+  int rc = f32(s->z1 + mh, bh, o);
+  if (rc == BD_OK) rc = b16(s->h1b + mh, bh, o + bh);
+  if (rc == BD_OK) rc = f32(s->z2 + mh, bh, o + 2 * bh);
+  if (rc == BD_OK) rc = b16(s->h2b + mh, bh, o + 3 * bh);
+  o += 4 * bh;
+  if (rc == BD_OK) rc = f32(s->y + mo, bo, o);
+  if (rc == BD_OK) rc = f32(s->dy + mo, bo, o + bo);
+  if (rc == BD_OK) rc = b16(s->dyb + mo, bo, o + 2 * bo);
+  o += 3 * bo;
+  if (rc == BD_OK) rc = f32(s->dz + mh, bh, o);
+  if (rc == BD_OK) rc = b16(s->dz2b + mh, bh, o + bh);
+  if (rc == BD_OK) rc = f32(s->dz1 + mh, bh, o + 2 * bh);
+  if (rc == BD_OK) rc = b16(s->dz1b + mh, bh, o + 3 * bh);
+  if (rc == BD_OK) MLP_TRY(hipStreamSynchronize(s->stream));
+  return rc;
 }
 
 // Device-resident training set: x[n][n_in], t[n][n_out].

@@ -86,6 +86,22 @@ class MLPEnsemble:
         check(self.lib.bd_mlp_gradients(self.h, _fp(x), _fp(t), len(x), _fp(g), _fp(loss)))
         return g, loss
 
+    def trace(self, member, batch):
+        """Intermediates of the last gradients/train_step call (test hook):
+        dict of z1, h1, z2, h2 (batch x hidden), y, dy, dyb (batch x n_out),
+        dz2, dz2b, dz1, dz1b (batch x hidden)."""
+        bh, bo = batch * self.hidden, batch * self.n_out
+        out = np.zeros(8 * bh + 3 * bo, dtype=np.float32)
+        check(self.lib.bd_mlp_trace(self.h, int(member), int(batch), _fp(out)))
+        names = [("z1", bh), ("h1", bh), ("z2", bh), ("h2", bh), ("y", bo), ("dy", bo),
+                 ("dyb", bo), ("dz2", bh), ("dz2b", bh), ("dz1", bh), ("dz1b", bh)]
+        res, k = {}, 0
+        for name, n in names:
+            cols = self.n_out if n == bo else self.hidden
+            res[name] = out[k:k + n].reshape(batch, cols).astype(np.float64)
+            k += n
+        return res
+
     def train_step(self, x, t, lr):
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)

@@ -404,6 +404,12 @@ int bd_mlp_forward(bd_mlp* s, const float* x, int32_t batch, float* y);
 /* mean-squared-error gradients of every member on one batch (test hook) */
 int bd_mlp_gradients(bd_mlp* s, const float* x, const float* t, int32_t batch, float* grads,
                      float* loss);
+/* test hook: member `member`'s intermediates of the last bd_mlp_gradients or
+ * bd_mlp_train_step call on `batch` rows, as floats in this order: z1, h1,
+ * z2, h2 [batch][hidden]; y, dy, dyb [batch][n_out]; dz2, dz2b, dz1, dz1b
+ * [batch][hidden] (h*, dyb and dz*b are the BF16 matrix-core operands, the
+ * others FP32): 8 batch hidden + 3 batch n_out values. */
+int bd_mlp_trace(bd_mlp* s, int32_t member, int32_t batch, float* out);
 int bd_mlp_train_step(bd_mlp* s, const float* x, const float* t, int32_t batch, float lr,
                       float* loss);
 /* device-resident training set and the full Adam loop over it */

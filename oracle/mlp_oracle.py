@@ -7,8 +7,10 @@ its matrix cores (inputs, weights, hidden activations, output and hidden
 gradients) are rounded to bfloat16 (round to nearest even) first, so the
 GPU result can be compared within FP32-accumulation tolerance.
 
-Pinned by: finite differences of the FP64 loss (gradients), and closed forms
-of GELU and of a one-step Adam update.
+Pinned by: finite differences of the FP64 loss (gradients), closed forms of
+GELU and of a one-step Adam update; the restatements from a GPU trace
+(stages_from_trace, grads_from_trace) by the FP64 pipeline itself on inputs
+and weights that are exact in bfloat16.
 """
 import numpy as np
 from scipy.special import erf
@@ -61,7 +63,10 @@ def forward(p, x, n_in, h, n_out, bf=False):
     return y, (x0, z1, h1, z2, h2)
 
 
-def loss_and_grads(p, x, t, n_in, h, n_out, bf=False):
+def loss_and_grads(p, x, t, n_in, h, n_out, bf=False, trace=False):
+    """Loss and packed gradients; with trace=True also the intermediates in
+    the layout of bd_mlp_trace (z1, h1, z2, h2, y, dy, dyb, dz2, dz2b, dz1,
+    dz1b; the *b entries as the matrix cores take them)."""
     d = unpack(p, n_in, h, n_out)
     q = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     y, (x0, z1, h1, z2, h2) = forward(p, x, n_in, h, n_out, bf)
@@ -75,7 +80,37 @@ def loss_and_grads(p, x, t, n_in, h, n_out, bf=False):
     dz1 = (q(dz2) @ q(d["W2"]).T) * gelu_grad(z1)
     g["W1"] = x0.T @ q(dz1)
     g["b1"] = dz1.sum(0)
+    if trace:
+        tr = {"z1": z1, "h1": h1, "z2": z2, "h2": h2, "y": y, "dy": dy, "dyb": q(dy),
+              "dz2": dz2, "dz2b": q(dz2), "dz1": dz1, "dz1b": q(dz1)}
+        return loss, pack(g, n_in, h, n_out), tr
     return loss, pack(g, n_in, h, n_out)
+
+
+def stages_from_trace(p, x, tr, n_in, h, n_out):
+    """Each stage of the GPU's forward and backward recomputed in FP64 from
+    the GPU's own inputs to that stage (tr: bd_mlp_trace's arrays): the
+    pre-activations z1, z2, y from the BF16 operands, and the FP32
+    pre-activation gradients dz2 = (dyb W3^T) gelu'(z2), dz1 = (dz2b W2^T)
+    gelu'(z1).  Only the accumulation order can differ from the GPU."""
+    d = unpack(p, n_in, h, n_out)
+    x0 = bf16(x)
+    return {"z1": x0 @ bf16(d["W1"]) + d["b1"],
+            "z2": tr["h1"] @ bf16(d["W2"]) + d["b2"],
+            "y": tr["h2"] @ bf16(d["W3"]) + d["b3"],
+            "dz2": (tr["dyb"] @ bf16(d["W3"]).T) * gelu_grad(tr["z2"]),
+            "dz1": (tr["dz2b"] @ bf16(d["W2"]).T) * gelu_grad(tr["z1"])}
+
+
+def grads_from_trace(x, tr, n_in, h, n_out):
+    """Parameter gradients from the GPU's own BF16 operands and FP32
+    pre-activation gradients (the products the GPU's matrix cores form), in
+    FP64: W3 = h2^T dyb, b3 = sum dy, W2 = h1^T dz2b, b2 = sum dz2,
+    W1 = bf16(x)^T dz1b, b1 = sum dz1."""
+    g = {"W3": tr["h2"].T @ tr["dyb"], "b3": tr["dy"].sum(0),
+         "W2": tr["h1"].T @ tr["dz2b"], "b2": tr["dz2"].sum(0),
+         "W1": bf16(x).T @ tr["dz1b"], "b1": tr["dz1"].sum(0)}
+    return pack(g, n_in, h, n_out)
 
 
 def adam(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
