@@ -10,10 +10,16 @@
 //   k_cv3d_hits  one thread per particle, candidates in ascending index
 //                order through LDS tiles; the particle's own hit row is
 //                accumulated in that order (Contracting_Vector3D.cu:85-131).
-//   k_cv3d_ata   SYRK-like 16x16 tiles of A^T A per axis, the sum over
-//                particles taken in ascending particle order so each entry
-//                is the same sequence of IEEE additions as the serial loop
-//                (Contracting_Vector3D.cu:135-148).
+//   k_cv3d_syrk  the normal equations as a SYRK of the augmented hit matrix
+//                [H | f] per axis: 64x64 tiles, 4x4 entries per thread, over
+//                blocks of 2048 particles taken in ascending order (the
+//                first block continuing the previous samples' sums);
+//                k_cv3d_reduce adds the blocks in ascending order.  With one
+//                block (n <= 2048) every entry is the serial loop's sequence
+//                of IEEE operations (Contracting_Vector3D.cu:135-156); with
+//                more, the same products summed block by block with fused
+//                multiply-adds (deterministic, within 1e-12 of the entry's
+//                scale sqrt(C_ii C_jj)).
 //   k_cv3d_hits_wave  the same rows with one wavefront per particle (bins <=
 //                2474), up to four per workgroup sharing LDS tiles of candidates: 64
 //                candidates tested per round, the round's hits added to an LDS
@@ -44,6 +50,8 @@ struct bd_cv3d {
   size_t hit_cap = 0;
   double* in = nullptr;   // 6 x n staging of host inputs
   size_t in_cap = 0;
+  double* part = nullptr;  // per particle block partial sums of the normal equations
+  size_t part_cap = 0;
   hipStream_t stream = nullptr;
   std::vector<double> result;
 };
@@ -51,7 +59,6 @@ struct bd_cv3d {
 namespace {
 
 constexpr int kHitThreads = 256;
-constexpr int kTile = 16;
 
 #define CV_TRY(expr)                                                                    \
   do {                                                                                  \
@@ -208,66 +215,126 @@ __global__ void __launch_bounds__(4 * 64)
   for (int k = lane; k < 3 * bins; k += 64) out[k] = row[k];
 }
 
-// One 16x16 tile (i-tile >= j-tile) of the normal equations of axis
-// blockIdx.z over the augmented columns 0..bins: column bins is the force
-// component, so the entries (bins, j) are A^T b (Contracting_Vector3D.cu:
-// 152-156) and the others A^T A (:135-148).  Each entry sums its products
-// over particles in ascending order.  The particles are taken 32 at a time
-// through double-buffered LDS, the next 32 rows loaded while these are used.
-constexpr int kChunk = 32;
+constexpr int kSyrkTile = 64;        // a workgroup's tile of the normal equations
+constexpr int kSyrkChunk = 16;       // particles staged in LDS at a time
+constexpr int kBlockParticles = 2048;  // particles per partial sum
 
 __device__ inline double aug_value(const double* __restrict__ hit, const double* __restrict__ f,
                                    int m, int a, int c, int bins) {
   return c < bins ? hit[((size_t)m * 3 + a) * bins + c] : (c == bins ? f[m] : 0.0);
 }
 
-__global__ void __launch_bounds__(kTile * kTile)
-    k_cv3d_ata(const double* __restrict__ hit, const double* __restrict__ fx,
-               const double* __restrict__ fy, const double* __restrict__ fz, int n, int bins,
-               double* __restrict__ ata, double* __restrict__ atb) {
-  const int ti = (int)blockIdx.y, tj = (int)blockIdx.x;
-  if (tj > ti) return;
-  const int a = (int)blockIdx.z;
+// Tile pair t (0 .. nt (nt + 1) / 2 - 1) -> (ti, tj), ti >= tj.
+__device__ inline void tile_pair(int t, int& ti, int& tj) {
+  ti = 0;
+  while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
+  tj = t - ti * (ti + 1) / 2;
+}
+
+// One 64x64 tile (tile pair blockIdx.x) of axis blockIdx.y over particle
+// block blockIdx.z: entries (i, j), i >= j, of the augmented normal
+// equations (rows 0..bins-1: A^T A, row bins: A^T b), each a running sum of
+// aug[m][i] * aug[m][j] over the block's particles in ascending order
+// (product, then sum).  Block 0 starts from the previous samples' sums.  With
+// one block (FMA false) the results are final (mirrored into the upper
+// triangle) and equal to the serial loop's; with several (FMA true: each
+// product and sum one fused operation, the summation order differing from
+// the serial one anyway) they go to the block's partial array.
+template <bool FMA>
+__global__ void __launch_bounds__(256)
+    k_cv3d_syrk(const double* __restrict__ hit, const double* __restrict__ fx,
+                const double* __restrict__ fy, const double* __restrict__ fz, int n, int bins,
+                int nblocks, double* __restrict__ ata, double* __restrict__ atb,
+                double* __restrict__ part) {
+  __shared__ double si[kSyrkChunk][kSyrkTile], sj[kSyrkChunk][kSyrkTile];
+  int ti, tj;
+  tile_pair((int)blockIdx.x, ti, tj);
+  const int a = (int)blockIdx.y, pb = (int)blockIdx.z;
   const double* f = a == 0 ? fx : (a == 1 ? fy : fz);
-  __shared__ double hi[2][kChunk][kTile + 1], hj[2][kChunk][kTile + 1];
-  const int li = (int)threadIdx.x / kTile, lj = (int)threadIdx.x % kTile;
-  const int i = ti * kTile + li, j = tj * kTile + lj;
-  const bool live = i <= bins && j < bins && j <= i;
+  const int naug = bins + 1;
+  const int ty = (int)threadIdx.x / 16, tx = (int)threadIdx.x % 16;
+  const int i0 = ti * kSyrkTile + ty * 4, j0 = tj * kSyrkTile + tx * 4;
   double* A = ata + (size_t)a * bins * bins;
-  double acc = live ? (i < bins ? A[(size_t)i * bins + j] : atb[(size_t)a * bins + j]) : 0.0;
-  // this thread loads rows li and li + 16 of a chunk, column lj of both tiles
-  double ri[2], rj[2];
-  auto fetch = [&](int m0) {
+  double acc[4][4];
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const int m = m0 + li + q * kTile;
-      ri[q] = m < n ? aug_value(hit, f, m, a, ti * kTile + lj, bins) : 0.0;
-      rj[q] = m < n ? aug_value(hit, f, m, a, tj * kTile + lj, bins) : 0.0;
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int i = i0 + r, j = j0 + c;
+      acc[r][c] = (pb == 0 && i < naug && j < bins && j <= i)
+                      ? (i < bins ? A[(size_t)i * bins + j] : atb[(size_t)a * bins + j])
+                      : 0.0;
     }
-  };
-  fetch(0);
-  for (int m0 = 0, buf = 0; m0 < n; m0 += kChunk, buf ^= 1) {
+  const int m_lo = pb * kBlockParticles, m_hi = min(n, m_lo + kBlockParticles);
+  // the staging of a chunk: thread t loads columns 4 (t % 16) .. + 3 of row t / 16
+  const int lr = (int)threadIdx.x / 16, lc = ((int)threadIdx.x % 16) * 4;
+  for (int m0 = m_lo; m0 < m_hi; m0 += kSyrkChunk) {
+    const int m = m0 + lr;
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-      hi[buf][li + q * kTile][lj] = ri[q];
-      hj[buf][li + q * kTile][lj] = rj[q];
+    for (int q = 0; q < 4; q++) {
+      si[lr][lc + q] = m < m_hi ? aug_value(hit, f, m, a, ti * kSyrkTile + lc + q, bins) : 0.0;
+      sj[lr][lc + q] = m < m_hi ? aug_value(hit, f, m, a, tj * kSyrkTile + lc + q, bins) : 0.0;
     }
     __syncthreads();
-    if (m0 + kChunk < n) fetch(m0 + kChunk);
-    const int mend = min(kChunk, n - m0);
-    if (live)
-      for (int k = 0; k < mend; k++) {
-        const double aux = hi[buf][k][li] * hj[buf][k][lj];
-        acc += aux;
-      }
-  }
-  if (live) {
-    if (i < bins) {
-      A[(size_t)i * bins + j] = acc;
-      A[(size_t)j * bins + i] = acc;
-    } else {
-      atb[(size_t)a * bins + j] = acc;
+    const int kend = min(kSyrkChunk, m_hi - m0);
+    for (int k = 0; k < kend; k++) {
+      double hi[4], hj[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) hi[r] = si[k][ty * 4 + r];
+#pragma unroll
+      for (int c = 0; c < 4; c++) hj[c] = sj[k][tx * 4 + c];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          if (FMA) {
+            acc[r][c] = fma(hi[r], hj[c], acc[r][c]);
+          } else {
+            const double aux = hi[r] * hj[c];
+            acc[r][c] += aux;
+          }
+        }
     }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int i = i0 + r, j = j0 + c;
+      if (!(i < naug && j < bins && j <= i)) continue;
+      if (nblocks > 1) {
+        part[(((size_t)pb * 3 + a) * naug + i) * naug + j] = acc[r][c];
+      } else if (i < bins) {
+        A[(size_t)i * bins + j] = acc[r][c];
+        A[(size_t)j * bins + i] = acc[r][c];
+      } else {
+        atb[(size_t)a * bins + j] = acc[r][c];
+      }
+    }
+}
+
+// The partial sums of the particle blocks added in ascending block order
+// (block 0 holds the previous samples' sums), one thread per entry (a, i, j),
+// i >= j, of the augmented lower triangle.
+__global__ void __launch_bounds__(256)
+    k_cv3d_reduce(const double* __restrict__ part, int nblocks, int bins,
+                  double* __restrict__ ata, double* __restrict__ atb) {
+  const int naug = bins + 1;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int a = (int)blockIdx.y;
+  const int i = (int)(e / naug), j = (int)(e % naug);
+  if (i >= naug || j >= bins || j > i) return;
+  const size_t stride = (size_t)3 * naug * naug;
+  const size_t k = ((size_t)a * naug + i) * naug + j;
+  double acc = part[k];
+  for (int pb = 1; pb < nblocks; pb++) acc += part[(size_t)pb * stride + k];
+  if (i < bins) {
+    double* A = ata + (size_t)a * bins * bins;
+    A[(size_t)i * bins + j] = acc;
+    A[(size_t)j * bins + i] = acc;
+  } else {
+    atb[(size_t)a * bins + j] = acc;
   }
 }
 
@@ -295,10 +362,34 @@ int accumulate(bd_cv3d* cv, const double* px, const double* py, const double* pz
     k_cv3d_hits<<<(n + kHitThreads - 1) / kHitThreads, kHitThreads, 0, st>>>(
         px, py, pz, n, side, periodic, cv->range * cv->range, bin_size_inv, cv->bins, cv->hit);
   CV_TRY(hipGetLastError());
-  const int nt = (cv->bins + 1 + kTile - 1) / kTile;
-  k_cv3d_ata<<<dim3(nt, nt, 3), kTile * kTile, 0, st>>>(cv->hit, fx, fy, fz, n, cv->bins,
-                                                        cv->ata, cv->atb);
-  CV_TRY(hipGetLastError());
+  const int naug = cv->bins + 1;
+  const int nt = (naug + kSyrkTile - 1) / kSyrkTile;
+  const int nblocks = (n + kBlockParticles - 1) / kBlockParticles;
+  if (nblocks > 1) {
+    const size_t pneed = (size_t)nblocks * 3 * naug * naug;
+    if (pneed > cv->part_cap) {
+      if (cv->part) (void)hipFree(cv->part);
+      cv->part = nullptr;
+      cv->part_cap = 0;
+      CV_TRY(hipMalloc((void**)&cv->part, pneed * sizeof(double)));
+      cv->part_cap = pneed;
+    }
+  }
+  if (n > 0) {
+    const dim3 grid(nt * (nt + 1) / 2, 3, nblocks);
+    if (nblocks > 1)
+      k_cv3d_syrk<true><<<grid, 256, 0, st>>>(cv->hit, fx, fy, fz, n, cv->bins, nblocks, cv->ata,
+                                              cv->atb, cv->part);
+    else
+      k_cv3d_syrk<false><<<grid, 256, 0, st>>>(cv->hit, fx, fy, fz, n, cv->bins, nblocks,
+                                               cv->ata, cv->atb, cv->part);
+    CV_TRY(hipGetLastError());
+    if (nblocks > 1) {
+      k_cv3d_reduce<<<dim3((unsigned)(((size_t)naug * naug + 255) / 256), 3), 256, 0, st>>>(
+          cv->part, nblocks, cv->bins, cv->ata, cv->atb);
+      CV_TRY(hipGetLastError());
+    }
+  }
   CV_TRY(hipStreamSynchronize(st));
   cv->samples++;
   cv->solved = false;
@@ -379,7 +470,7 @@ int bd_cv3d_create(int32_t bins, double range, bd_cv3d** out) {
 int bd_cv3d_destroy(bd_cv3d* cv) {
   if (!cv) return BD_OK;
   if (cv->stream) (void)hipStreamSynchronize(cv->stream);
-  double* p[] = {cv->ata, cv->atb, cv->hit, cv->in};
+  double* p[] = {cv->ata, cv->atb, cv->hit, cv->in, cv->part};
   for (double* q : p)
     if (q) (void)hipFree(q);
   if (cv->stream) (void)hipStreamDestroy(cv->stream);

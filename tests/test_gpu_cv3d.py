@@ -43,6 +43,38 @@ def test_normal_equations_bit_exact_host_inputs():
     c.close()
 
 
+def test_normal_equations_in_particle_blocks_within_tolerance():
+    """Above 2048 particles the normal equations are summed per block of 2048
+    particles and the blocks added in order: every entry within 1e-12 of its
+    Cauchy-Schwarz scale sqrt(C_ii C_jj) (the bound of a sum of h_i h_j over
+    particles) of the oracle's serial sums, over two samples; the solution
+    within 1e-9."""
+    rng = np.random.default_rng(11)
+    bins, rr, n, side = 100, 2.5, 8192, 19.0
+    c = Contraction(bins, rr)
+    A = np.zeros((3, bins, bins))
+    B = np.zeros((3, bins))
+    ff = np.zeros(3)
+    for _ in range(2):
+        x = rng.random((n, 3)) * side
+        f = rng.normal(size=(n, 3))
+        c.add_host(x, f, side)
+        assert orc.lib().orc_cv3d_accumulate(bins, rr, orc.dp(np.ascontiguousarray(x)),
+                                             orc.dp(np.ascontiguousarray(f)), n, side,
+                                             orc.dp(A), orc.dp(B)) == 0
+        ff += (f ** 2).sum(0)
+        ga, gb = c.normal()
+        for a in range(3):
+            d = np.sqrt(np.diag(A[a]))
+            assert np.all(np.abs(ga[a] - A[a]) <= 1e-12 * np.outer(d, d) + 1e-300)
+            assert np.all(np.abs(gb[a] - B[a]) <= 1e-12 * np.sqrt(ff[a]) * d + 1e-300)
+        assert not np.array_equal(ga, A)  # the blocks' order is not the serial one
+    c.solve()
+    exp = orc.cv3d_solve(A, B)
+    assert np.allclose(c.vector(), exp, rtol=1e-9, atol=1e-9 * np.abs(exp).max())
+    c.close()
+
+
 def test_many_bins_take_the_thread_per_particle_rows():
     """Beyond 2474 bins a hit row does not fit 64 KB of LDS and the rows are
     built one thread per particle: same normal equations as the oracle."""
