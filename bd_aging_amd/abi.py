@@ -126,6 +126,7 @@ SIGNATURES = {
     "bd_lane_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
     "bd_random_positions": (C.c_int, [C.c_int32, C.c_double, C.c_uint64, _D]),
     "bd_gder": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, _U]),
+    "bd_max_occupancy": (C.c_int, [_P, C.c_int32, _I, _I]),
     "bd_observables": (C.c_int, [_P, C.c_int32, C.c_double, C.POINTER(BDObs)]),
     "bd_gder_all": (C.c_int, [_P, C.c_int32, C.c_double, _U]),
     "bd_gder_noalloc": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, _U]),

@@ -678,20 +678,7 @@ int main(int argc, char** argv) {
       std::fill(counts.begin(), counts.end(), 0);
       check(bd_gder(s, r, nbins_gder, range_gder, counts.data()), "bd_gder");
       for (size_t q = 0; q < counts.size(); q++) gders[(size_t)k * 4 * nbins_gder + q] += (double)counts[q];
-      if (r == 0) {
-        bd_geometry gr;
-        check(bd_get_geometry(s, r, &gr), "geometry");
-        std::vector<int32_t> nocup, cv;
-        for (int sp = 0; sp < 2; sp++) {
-          const int nc = sp == 0 ? gr.ncell_big : gr.ncell_sml;
-          const int nt = sp == 0 ? ntags_big : ntags_sml;
-          nocup.resize((size_t)nc * nc * nc);
-          cv.resize((size_t)nc * nc * nc * nt);
-          check(bd_get_cells(s, r, sp, nocup.data(), cv.data()), "bd_get_cells");
-          const int mx = *std::max_element(nocup.begin(), nocup.end());
-          (sp == 0 ? ocb : ocs) = mx;
-        }
-      }
+      if (r == 0) check(bd_max_occupancy(s, r, &ocb, &ocs), "bd_max_occupancy");
       // decade dumps (main.cu:809-835)
       if (log_mode && k % points_per_decade == 1 && print_decades_configs &&
           number_of_tw <= number_of_tws) {

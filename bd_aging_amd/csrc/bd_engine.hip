@@ -3970,6 +3970,24 @@ int bd_get_cells(bd_sys* s, int32_t replica, int32_t species, int32_t* nocup,
   return BD_OK;
 }
 
+int bd_max_occupancy(bd_sys* s, int32_t replica, int32_t* big, int32_t* sml) {
+  int rc = check_rep(s, replica);
+  if (rc != BD_OK) return rc;
+  if (!big || !sml) return fail(BD_E_ARG, "null output");
+  if (!s->located) return fail(BD_E_STATE, "no binning yet");
+  const RepDev& R = s->reps[replica];
+  int32_t* out[2] = {big, sml};
+  for (int g = 0; g < 2; g++) {
+    const size_t nc3 = (size_t)R.ncell[g] * R.ncell[g] * R.ncell[g];
+    std::vector<int32_t> occ(nc3);
+    HIP_TRY(hipMemcpyAsync(occ.data(), s->cnt + R.cell_off[g], sizeof(int32_t) * nc3,
+                           hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    *out[g] = nc3 ? *std::max_element(occ.begin(), occ.end()) : 0;
+  }
+  return BD_OK;
+}
+
 int bd_compute_forces(bd_sys* s) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (!s->located) return fail(BD_E_STATE, "bd_cell_locate must run first");

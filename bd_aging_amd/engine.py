@@ -164,6 +164,12 @@ class Engine:
     def cell_locate(self):
         check(self.lib.bd_cell_locate(self.h))
 
+    def max_occupancy(self, r=0):
+        """(big, small) largest cell occupancy after the last binning."""
+        b, s = C.c_int32(), C.c_int32()
+        check(self.lib.bd_max_occupancy(self.h, r, C.byref(b), C.byref(s)))
+        return b.value, s.value
+
     def cells(self, species, r=0):
         g = self.geometry(r)
         p = self.params[r]

@@ -148,6 +148,10 @@ int bd_cell_locate(bd_sys* s);
 /* Export one grid in the reference "tags" layout: nocup[ncell^3] and
  * cell_vec[ncell^3 * ntags] (species-local indices, -1 in empty slots). */
 int bd_get_cells(bd_sys* s, int32_t replica, int32_t species, int32_t* nocup, int32_t* cell_vec);
+/* Largest occupancy of a cell in each grid of one replica after the last
+ * binning: the nocup maxima main.cu:641-650 scans on the host for its log
+ * line (the occupancies only, no cell lists). */
+int bd_max_occupancy(bd_sys* s, int32_t replica, int32_t* big, int32_t* sml);
 /* Forces, virials and potentials from the current cells (get_forces.cu). */
 int bd_compute_forces(bd_sys* s);
 /* Neighbour lists in traversal order (test hook): counts[N], list[N*maxn]

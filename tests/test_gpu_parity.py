@@ -70,6 +70,17 @@ def test_cells_bit_exact(config1_positions, pot):
     e.close()
 
 
+def test_max_occupancy_equals_cell_lists(config1_positions):
+    side, x = config1_positions
+    e = Engine(make_params(1000, 1000, side=side, seed=4))
+    e.set_positions(x)
+    with pytest.raises(BDError):
+        e.max_occupancy()  # no binning yet
+    e.step(3)
+    assert e.max_occupancy() == (e.cells(0)[0].max(), e.cells(1)[0].max())
+    e.close()
+
+
 def test_cells_edge_positions():
     p = make_params(200, 200, phi=0.1)
     rng = np.random.default_rng(1)
