@@ -2692,6 +2692,14 @@ __global__ void __launch_bounds__(256)
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) b[k] = a[k];
 }
 
+// Grid barriers only, for the barrier-cost probe: every workgroup of a
+// cooperative launch passes `barriers` grid-wide barriers.
+__global__ void __launch_bounds__(kBlk) k_grid_barrier(int barriers, int* __restrict__ done) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  for (int k = 0; k < barriers; k++) grid.sync();
+  if (blockIdx.x == 0 && threadIdx.x == 0) done[0] = barriers;
+}
+
 // Stand-alone rannew64 lanes for the stream test.
 __global__ void k_rng_stream(uint64_t seed, int lanes, int ndraws, uint64_t* __restrict__ ring,
                              uint64_t* __restrict__ out) {
@@ -4600,6 +4608,51 @@ int bd_stream_bandwidth(int64_t bytes, int32_t reps, double* gbs) {
   if (e != hipSuccess)
     return fail(BD_E_HIP, std::string("bd_stream_bandwidth: ") + hipGetErrorString(e));
   *gbs = 2.0 * (double)(n * sizeof(double4)) * reps / (total_ms * 1e-3) / 1e9;
+  return BD_OK;
+}
+
+int bd_grid_barrier_cost(int32_t workgroups, int32_t barriers, double* us,
+                         int32_t* max_workgroups) {
+  if (workgroups < 1 || barriers < 1 || barriers > 1000000 || !us)
+    return fail(BD_E_ARG, "bad arguments");
+  const void* fn = reinterpret_cast<const void*>(&k_grid_barrier);
+  int dev = 0, coop = 0, ncu = 0, per_cu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  if (!coop) return fail(BD_E_STATE, "the device does not support cooperative launches");
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlk, 0));
+  if (max_workgroups) *max_workgroups = per_cu * ncu;
+  // a cooperative grid must be co-resident, or its barriers would never end
+  if (workgroups > per_cu * ncu)
+    return fail(BD_E_ARG, "more workgroups than are co-resident on the device");
+  int* done = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&done, sizeof(int));
+  if (e == hipSuccess) e = hipStreamCreate(&st);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int nb = barriers;
+  void* args[] = {&nb, &done};
+  double ms = 0.0;
+  for (int k = 0; k < 2 && e == hipSuccess; k++) {  // launch 0 warms up
+    e = hipEventRecord(e0, st);
+    if (e == hipSuccess)
+      e = hipLaunchCooperativeKernel(fn, dim3(workgroups), dim3(kBlk), args, 0, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    ms = (double)t;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (done) (void)hipFree(done);
+  if (e != hipSuccess)
+    return fail(BD_E_HIP, std::string("bd_grid_barrier_cost: ") + hipGetErrorString(e));
+  *us = ms * 1e3 / barriers;
   return BD_OK;
 }
 

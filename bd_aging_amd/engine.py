@@ -364,6 +364,15 @@ def stream_bandwidth(nbytes=1 << 30, reps=10):
     return v.value
 
 
+def grid_barrier_cost(workgroups, barriers=2000):
+    """Microseconds per grid-wide barrier of a cooperative launch of
+    `workgroups` 64-thread workgroups that does nothing else, and the
+    co-resident maximum of such workgroups."""
+    us, mx = C.c_double(), C.c_int32()
+    check(load().bd_grid_barrier_cost(int(workgroups), int(barriers), C.byref(us), C.byref(mx)))
+    return us.value, mx.value
+
+
 class Contraction:
     """Force contraction (Contracting_Vector3D): add samples, solve, read."""
 

@@ -24,16 +24,19 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from bd_aging_amd.engine import Contraction, Engine, make_params, box_side, stream_bandwidth  # noqa: E402,E501
+from bd_aging_amd.engine import (Contraction, Engine, make_params, box_side,  # noqa: E402
+                                 grid_barrier_cost, stream_bandwidth)
 from bd_aging_amd.init import relax_overlaps, relax_batch  # noqa: E402
 
 HBM_PEAK = 8.0e12          # B/s, MI355X
-FP64_PEAK = 78.6e12        # FLOP/s, MI355X vector FP64
 B_FORCES = 72              # algorithmic bytes per particle, force pass
 B_BIN = 36                 # algorithmic bytes per particle, binning pass
 B_UPDATE = 120             # algorithmic bytes per particle, update pass (ring excluded)
 B_STEP = 228               # algorithmic bytes per particle-step (bin 36 + forces 72 + update 120)
-FLOP_PAIR = 22             # FP64 operations per ordered pair in the g(r) scan
+# g(r) binnings timed: config 1's (200 bins to L/2), the batch's (128 bins to
+# 5 sigma) and the reference script's (400 bins to 15)
+GDER_CASES = (("200 bins to L/2", 200, None), ("128 bins to 5", 128, 5.0),
+              ("400 bins to 15", 400, 15.0))
 CV3D_BINS, CV3D_RANGE = 400, 5.0   # force-contraction sample: bins to range
 RNG_BYTES = 112            # rannew64 ring and LCG bytes per particle-step (8 x 8 B read,
                            # 4 x 8 B written, LCG 16 B), beside the update's 120 B
@@ -95,7 +98,7 @@ def timed_launch_mode(eng, steps, graph_steps):
             "fused_update": fused, "graph_replays": replays, "eager_steps": eager, "note": note}
 
 
-def run_plain(steps, warmup, graph_steps=16, shape=None):
+def run_plain(steps, warmup, graph_steps=16, shape=None, persistent_steps=0):
     nb = ns = 16384
     side = box_side(nb, ns, 1.4, 1.0, 0.58)
     log(f"[bench] relaxing N=32768 phi=0.58 L={side:.4f}")
@@ -121,13 +124,8 @@ def run_plain(steps, warmup, graph_steps=16, shape=None):
     if eng.fused_update():
         fused_ms = (eng.kernel_times_fused(ksteps + 1) / [ksteps + 1, ksteps]).tolist()
     obs = eng.observables(7.0)
-    eng.gder_counts(128, 5.0)  # warm-up
-    t0 = time.perf_counter()
+    gder = time_gder(eng, side, nb + ns, reps=3)
     reps = 5
-    for _ in range(reps):
-        gcounts = eng.gder_counts(128, 5.0)
-    gder_s = (time.perf_counter() - t0) / reps
-    gder_hits = int(np.asarray(gcounts).sum())
     # force contraction (Contracting_Vector3D): one sample of the current state
     cv = Contraction(CV3D_BINS, CV3D_RANGE)
     cv.add(eng)  # warm-up
@@ -136,11 +134,59 @@ def run_plain(steps, warmup, graph_steps=16, shape=None):
         cv.add(eng)
     cv3d_s = (time.perf_counter() - t0) / reps
     cv.close()
+    persistent = None
+    if persistent_steps > 0:
+        persistent = persistent_compare(eng, persistent_steps, graph_steps)
     eng.close()
     return {"N": nb + ns, "side": side, "seconds": dt, "rate": (nb + ns) * steps / dt,
             "kernel_ms": ms.tolist(), "obs": obs, "x": x, "params": p, "shape": shape,
             "mode": mode,
-            "gder_s": gder_s, "gder_hits": gder_hits, "fused_ms": fused_ms, "cv3d_s": cv3d_s}
+            "gder": gder, "fused_ms": fused_ms, "cv3d_s": cv3d_s, "persistent": persistent}
+
+
+def persistent_compare(eng, steps, graph_steps):
+    """The same system stepped `steps` times by one launch of the persistent
+    cooperative step kernel (three grid barriers per step) and by graph
+    replays, and the cost of one grid barrier alone at several grid sizes."""
+    eng.set_persistent(steps)
+    eng.step(1)  # warm-up launch
+    t0 = time.perf_counter()
+    eng.step(steps)
+    t_p = time.perf_counter() - t0
+    eng.set_persistent(0)
+    timed_launch_mode(eng, steps, graph_steps)
+    t0 = time.perf_counter()
+    eng.step(steps)
+    t_g = time.perf_counter() - t0
+    _, most = grid_barrier_cost(64, 10)
+    barrier = {str(g): round(grid_barrier_cost(g)[0], 2) for g in (64, 256, 512, 1024, 2048, most)
+               if g <= most}
+    log(f"[bench] persistent {t_p / steps * 1e6:.1f} us/step, graphs {t_g / steps * 1e6:.1f} "
+        f"us/step; grid barrier us by workgroups {barrier}")
+    return {"steps": steps, "us_per_step_persistent": t_p / steps * 1e6,
+            "us_per_step_graphs": t_g / steps * 1e6, "grid_barrier_us": barrier,
+            "coresident_barrier_workgroups": most}
+
+
+def time_gder(eng, side, N, reps=3):
+    """Pair-histogram rate at each binning of GDER_CASES: ordered pairs per
+    second (N (N - 1) per call), the fraction of pairs within range (each one
+    square root, one bin and one LDS increment) and the LDS increments per
+    second."""
+    out = []
+    for name, nbins, rng in GDER_CASES:
+        r = 0.5 * side if rng is None else rng
+        eng.gder_counts(nbins, r)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            c = eng.gder_counts(nbins, r)
+        dt = (time.perf_counter() - t0) / reps
+        hits = int(np.asarray(c).sum())
+        pairs = N * (N - 1)
+        out.append({"binning": name, "bins": nbins, "range": round(r, 4), "ms": dt * 1e3,
+                     "pairs_per_s": pairs / dt, "in_range_fraction": hits / pairs,
+                     "lds_atomics_per_s": hits / dt})
+    return out
 
 
 def run_batch(steps, warmup, graph_steps=16, shape=None):
@@ -195,11 +241,7 @@ def run_stress(steps, warmup, graph_steps=16, shape=None):
     dt = time.perf_counter() - t0
     ms = eng.kernel_times(100) / 100
     geo = eng.geometry()
-    eng.gder_counts(128, 5.0)  # warm-up
-    t0 = time.perf_counter()
-    gcounts = eng.gder_counts(128, 5.0)
-    gder_s = time.perf_counter() - t0
-    gder_hits = int(np.asarray(gcounts).sum())
+    gder = time_gder(eng, side, nb + ns, reps=1)
     eng.close()
     N = nb + ns
     kern = {}
@@ -211,13 +253,7 @@ def run_stress(steps, warmup, graph_steps=16, shape=None):
     kern["update"]["GB_s_with_rng"] = (B_UPDATE + RNG_BYTES) * N / (ms[0] * 1e-3) / 1e9
     return {"N": N, "side": round(side, 4), "steps": steps, "rate": N * steps / dt,
             "cells": f"{geo.ncell_big}^3 big + {geo.ncell_sml}^3 small",
-            "force_kernel": shape, "kernels": kern,
-            "gder": {"pairs_per_s": N * (N - 1) / gder_s,
-                     "GFLOP_s": FLOP_PAIR * N * (N - 1) / gder_s / 1e9,
-                     "fp64_frac": FLOP_PAIR * N * (N - 1) / gder_s / FP64_PEAK,
-                     # one LDS atomic increment per ordered pair within range
-                     "lds_atomics_per_s": gder_hits / gder_s,
-                     "bins": 128, "range": 5.0}}
+            "force_kernel": shape, "kernels": kern, "gder": gder}
 
 
 def host_info():
@@ -307,6 +343,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stress-steps", type=int, default=1000,
                     help="steps of the N=262144 stress configuration (0 skips it)")
+    ap.add_argument("--persistent-steps", type=int, default=200,
+                    help="steps of the persistent-kernel comparison on the plain system "
+                         "(0 skips it)")
     ap.add_argument("--shape", default=None,
                     help="force kernel as BRICK,DEPTH[,WALK[,SPLIT]] (e.g. 0,16,3,2) instead of "
                          "autotuning")
@@ -330,7 +369,7 @@ def main():
         import multiprocessing as mp
         workers = max(1, min(16, os.cpu_count() or 1))
         pool = mp.get_context("fork").Pool(workers)
-    plain = run_plain(a.steps, a.warmup, a.graph_steps, shape)
+    plain = run_plain(a.steps, a.warmup, a.graph_steps, shape, a.persistent_steps)
     log(f"[bench] plain: {plain['rate']:.4e} particle-steps/s, kernel ms {plain['kernel_ms']}")
     batch = None
     if not a.no_batch:
@@ -359,7 +398,9 @@ def main():
     achieved = f_bytes * N / (f_ms * 1e-3)
     copy_gbs = stream_bandwidth()
     log(f"[bench] device copy kernel: {copy_gbs:.0f} GB/s")
-    cv3d_flop = FLOP_PAIR * N * (N - 1) + 3 * CV3D_BINS * (CV3D_BINS + 1) * N
+    # the normal equations' multiply-adds: 3 axes x the (bins + 1)-column
+    # augmented lower triangle x N particles
+    cv3d_macs = 3 * (CV3D_BINS + 1) * (CV3D_BINS + 2) // 2 * N
     out = {
         "metric": "particle-steps/s at N=32768 phi=0.58 (plain single system)",
         "value": plain["rate"],
@@ -387,15 +428,11 @@ def main():
                       "forces": plain["kernel_ms"][2],
                       "forces_with_update": plain["fused_ms"][1] if plain["fused_ms"] else None},
         "step_hbm_frac": B_STEP * plain["rate"] / HBM_PEAK,
-        "gder": {"pairs_per_s": N * (N - 1) / plain["gder_s"],
-                 "GFLOP_s": FLOP_PAIR * N * (N - 1) / plain["gder_s"] / 1e9,
-                 "fp64_frac": FLOP_PAIR * N * (N - 1) / plain["gder_s"] / FP64_PEAK,
-                 "lds_atomics_per_s": plain["gder_hits"] / plain["gder_s"],
-                 "flop_per_pair": FLOP_PAIR, "bins": 128, "range": 5.0},
+        "gder": plain["gder"],
+        "persistent": plain["persistent"],
         "cv3d": {"bins": CV3D_BINS, "range": CV3D_RANGE, "ms_per_sample": plain["cv3d_s"] * 1e3,
-                 "GFLOP_s": cv3d_flop / plain["cv3d_s"] / 1e9,
-                 "fp64_frac": cv3d_flop / plain["cv3d_s"] / FP64_PEAK,
-                 "flop_model": "22 per ordered pair + 3 bins (bins + 1) per particle"},
+                 "pairs_per_s": N * (N - 1) / plain["cv3d_s"],
+                 "normal_equation_macs": cv3d_macs},
         "cpu_baseline": cpu,
     }
     if stress:

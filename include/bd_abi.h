@@ -233,12 +233,14 @@ int bd_set_walk(bd_sys* s, int32_t walk);
 int bd_get_walk(const bd_sys* s, int32_t* walk);
 
 /* Neighbour lists (BD_WALK_LIST, no reference counterpart): each particle
- * lists every particle within rc + skin; the lists of a replica are rebuilt
- * from the cells on the first force evaluation after a move took some
- * particle more than skin/2 from its position at the last build.  Between
- * rebuilds a step tests only the listed candidates, and sorts the partners
- * inside the cutoff into the cell walk's order (get_forces.cu:21-558, cells
- * in (di, dj, dk) order, ids ascending) before adding their terms:
+ * lists every particle within rc + skin.  A move that takes some particle of
+ * a replica more than skin/2 from its position at the last build flags the
+ * replica: on the next force evaluation that replica walks its cells while
+ * a build kernel, on a second stream forked from and joined back into the
+ * step, rebuilds its lists for the evaluations after.  Between rebuilds a
+ * step tests only the listed candidates, and sorts the partners inside the
+ * cutoff into the cell walk's order (get_forces.cu:21-558, cells in
+ * (di, dj, dk) order, ids ascending) before adding their terms:
  * bit-identical to the cell walks.  Replicas whose grids are too small for
  * the list stencil, and particles with more than 48 candidates or 32
  * partners, walk their cells.  The update runs as its own kernel with this
@@ -325,6 +327,15 @@ int bd_kernel_times_fused(bd_sys* s, int32_t nsteps, double* ms);
  * plus bytes written per second, in GB/s (the achievable peak beside the
  * data-sheet figure). */
 int bd_stream_bandwidth(int64_t bytes, int32_t reps, double* gbs);
+/* Grid-barrier probe (no reference counterpart): one cooperative launch of
+ * `workgroups` 64-thread workgroups that pass `barriers` grid-wide barriers
+ * and nothing else, after a warm-up launch; *us = microseconds per barrier
+ * (runtime-stamped events around the launch).  *max_workgroups (may be NULL)
+ * receives how many such workgroups are co-resident on the device; more
+ * than that is BD_E_ARG, since a cooperative grid must be co-resident.  The
+ * cost that the persistent step kernel pays three times per step. */
+int bd_grid_barrier_cost(int32_t workgroups, int32_t barriers, double* us,
+                         int32_t* max_workgroups);
 
 /* ---- rannew64 lanes ---- */
 /* State of one lane of a replica: ring[250] (slot order) and the LCG word;

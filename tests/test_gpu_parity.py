@@ -898,3 +898,16 @@ def test_stream_bandwidth_probe():
     assert 100.0 < gbs < 20000.0
     with pytest.raises(BDError):
         stream_bandwidth(1024, 3)
+
+
+def test_grid_barrier_probe():
+    """The barrier probe times co-resident grids, costs more with more
+    workgroups, and refuses a grid that could not be co-resident."""
+    from bd_aging_amd.engine import grid_barrier_cost
+    us64, most = grid_barrier_cost(64, 200)
+    assert most >= 256 and 0.0 < us64 < 1000.0
+    us_most, _ = grid_barrier_cost(most, 200)
+    assert us_most > us64
+    with pytest.raises(BDError) as e:
+        grid_barrier_cost(most + 1, 10)
+    assert e.value.code == abi.BD_E_ARG
