@@ -190,6 +190,56 @@ def test_every_brick_shape_bit_identical(config1_positions, pot):
     e.close()
 
 
+def test_pairs_at_the_cutoff():
+    """Pairs within 1e-9 of the cutoff on both sides, directly and through
+    the periodic faces, for all three species pairs: every walk keeps
+    exactly the oracle's partners (a pair term just beyond the cutoff is
+    tiny but not zero, so one pair too many changes the bits)."""
+    side = 13.0
+    rng = np.random.default_rng(17)
+    p = make_params(12, 12, side=side, seed=5)
+    e = Engine(p)
+    g = e.geometry()
+    # grids of at least 2 nrange + 3 cells: the slab walks run (not the generic one)
+    assert g.ncell_big >= 2 * max(g.nrange_bb, g.nrange_sb) + 3
+    assert g.ncell_sml >= 2 * max(g.nrange_ss, g.nrange_bs) + 3
+    rc = {"bb": g.rc_bb, "ss": g.rc_ss, "bs": g.rc_bs}
+    x = np.empty((24, 3))
+    x[:12] = rng.uniform(0, side, (12, 3))
+    x[12:] = rng.uniform(0, side, (12, 3))
+    # (i, j, cutoff, signed offset of r^2 from rc^2, axis, across a face)
+    plan = [(0, 1, "bb", -5e-10, 0, False), (2, 3, "bb", 5e-10, 1, True),
+            (12, 13, "ss", 4e-10, 2, True), (14, 15, "ss", -4e-10, 0, True),
+            (4, 16, "bs", 3e-10, 1, False), (5, 17, "bs", -3e-10, 2, True),
+            (6, 18, "bs", 0.0, 0, True), (7, 8, "bb", 0.0, 2, False)]
+    for i, j, kind, off, axis, face in plan:
+        r = np.sqrt(rc[kind] ** 2 + off)
+        xi = rng.uniform(3.0, side - 3.0, 3)
+        if face:
+            xi[axis] = 0.3  # the partner sits across the x = 0 (or y, z) face
+        xj = xi.copy()
+        xj[axis] = (xi[axis] - r) % side
+        x[i], x[j] = xi, xj
+    o = orc.System(p)
+    o.set_positions(x)
+    o.cell_locate()
+    o.compute_forces()
+    ref = o.forces()
+    counts, _ = o.neighbours(64)
+    assert counts.sum() > 0
+    e.set_positions(x)
+    e.cell_locate()
+    for walk, depth, split in ((abi.BD_WALK_SLABS, 16, 0), (abi.BD_WALK_ROWS, 0, 0),
+                               (abi.BD_WALK_CHUNK4, 16, 2), (abi.BD_WALK_CHUNK2, 32, 1)):
+        e.set_walk(walk)
+        e.set_defer_depth(depth)
+        e.set_pass_split(split)
+        e.compute_forces()
+        assert all(np.array_equal(a, b) for a, b in zip(e.forces(), ref)), (walk, depth, split)
+    e.close()
+    o.close()
+
+
 def test_pass_split_column_overflow(config1_positions):
     """A big particle with more cross-species partners than the deferral
     column holds (40 small particles packed around it): the split kernel
@@ -217,7 +267,7 @@ def test_pass_split_column_overflow(config1_positions):
     assert all(np.array_equal(a, b) for a, b in zip(e.forces(), (fo, vo, uo)))
     for walk in (abi.BD_WALK_SLABS, abi.BD_WALK_CHUNK2, abi.BD_WALK_CHUNK4):
         for depth in (16, 32):
-            for split in (1, 2):
+            for split in (0, 1, 2):
                 e.set_walk(walk)
                 e.set_defer_depth(depth)
                 e.set_pass_split(split)
