@@ -30,6 +30,7 @@ BD_WALK_SLABS = 1
 BD_WALK_CHUNK2 = 2
 BD_WALK_CHUNK4 = 3
 BD_WALK_LIST = 4
+BD_WALK_FLAT = 5
 BD_TEMP_LINEAR = 0
 BD_TEMP_SINE = 1
 
@@ -106,6 +107,7 @@ SIGNATURES = {
     "bd_get_graph_steps": (C.c_int, [_P, _I]),
     "bd_set_walk": (C.c_int, [_P, C.c_int32]),
     "bd_get_walk": (C.c_int, [_P, _I]),
+    "bd_set_flat_depth": (C.c_int, [_P, C.c_int32]),
     "bd_set_pass_split": (C.c_int, [_P, C.c_int32]),
     "bd_set_list_skin": (C.c_int, [_P, C.c_double]),
     "bd_get_list_skin": (C.c_int, [_P, C.POINTER(C.c_double)]),

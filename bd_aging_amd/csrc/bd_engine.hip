@@ -812,7 +812,8 @@ __device__ inline void pair_add(const RepDev& R, double sigma, double dist2, dou
 }
 
 // Deferred pair terms: the walk records each partner inside the cutoff as
-// slot | image codes (2 bits per axis: 0 none, 1 -L, 2 +L) | pass bit in a
+// slot | image codes (2 bits per axis, two's complement: 0 none, 1 +L,
+// 3 -L) | pass bit in a
 // per-thread LDS column, for both stencil passes of the particle; drain()
 // evaluates the recorded terms in recording order, recomputing the
 // separation with the same operations.  The potential is then evaluated by
@@ -823,7 +824,16 @@ constexpr unsigned int kSlotMask = (1u << kSlotBits) - 1u;
 constexpr unsigned int kPassBit = 1u << 31;
 
 __device__ inline unsigned int image_code(double sh) {
-  return sh == 0.0 ? 0u : (sh < 0.0 ? 1u : 2u);
+  return sh == 0.0 ? 0u : (sh < 0.0 ? 3u : 1u);
+}
+
+// The shift of the 2-bit image code at bit `bit` of `word`: L times the
+// code's sign-extended value (-1, 0, +1).  L * -1 = -L and L * 0 = +0 exactly,
+// so the shift equals the one the code was made from (a zero shift at most
+// flips the sign of a zero difference, which no squared distance, force or
+// virial term can see), in three instructions (extract, convert, multiply).
+__device__ inline double image_at(unsigned int word, int bit, double L) {
+  return L * (double)((int)(word << (30 - bit)) >> 30);
 }
 
 struct Defer {
@@ -843,11 +853,9 @@ __device__ inline void drain(const RepDev& R, double x, double y, double z, Defe
   for (int e = 0; e < nd; e++) {
     const unsigned int wc = d.buf[e * kBlk];
     const double4 qc = ((wc & kPassBit) ? d.cp1 : d.cp0)[wc & kSlotMask];
-    const unsigned int cx = (wc >> kSlotBits) & 3u, cy = (wc >> (kSlotBits + 2)) & 3u,
-                       cz = (wc >> (kSlotBits + 4)) & 3u;
-    const double dx = (qc.x - x) + (cx == 0u ? 0.0 : (cx == 1u ? -L : L));
-    const double dy = (qc.y - y) + (cy == 0u ? 0.0 : (cy == 1u ? -L : L));
-    const double dz = (qc.z - z) + (cz == 0u ? 0.0 : (cz == 1u ? -L : L));
+    const double dx = (qc.x - x) + image_at(wc, kSlotBits, L);
+    const double dy = (qc.y - y) + image_at(wc, kSlotBits + 2, L);
+    const double dz = (qc.z - z) + image_at(wc, kSlotBits + 4, L);
     const double dist2 = dx * dx + dy * dy + dz * dz;
     pair_add<POT>(R, (wc & kPassBit) ? d.sigma1 : d.sigma0, dist2, dx, dy, dz, fx, fy, fz, vir,
                   pot);
@@ -979,10 +987,6 @@ __device__ inline void scan_rows(const RepDev& R, int n, double cs, double sigma
 // most of its lanes.
 constexpr int kSlabSegs = 10;  // 2 ranges x (2 NR + 1) rows, NR <= 2
 
-__device__ inline double image_of(unsigned int code, double L) {
-  return code == 0u ? 0.0 : (code == 1u ? -L : L);
-}
-
 template <int POT, int NR, int DEPTH, bool REC = false>
 __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigma, double rc2,
                                   int self, double x, double y, double z, const int hw[3],
@@ -1032,7 +1036,7 @@ __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigm
         if (z0 < 0) {
           s0a = rrow[z0 + n];
           s1a = rrow[n];
-          za = 1u;
+          za = 3u;  // -L
           s0b = rrow[0];
           s1b = rrow[z1 + 1];
         } else if (z1 >= n) {
@@ -1040,7 +1044,7 @@ __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigm
           s1a = rrow[n];
           s0b = rrow[0];
           s1b = rrow[z1 - n + 1];
-          zb = 2u;
+          zb = 1u;  // +L
         } else {
           s0a = rrow[z0];
           s1a = rrow[z1 + 1];
@@ -1089,8 +1093,8 @@ __device__ inline void scan_slabs(const RepDev& R, int n, double cs, double sigm
       }
       if (more) qn = cp[st + t];
       const double dx = (q.x - x) + shx;
-      const double dy = (q.y - y) + image_of((mq >> 16) & 3u, L);
-      const double dz = (q.z - z) + image_of((mq >> 18) & 3u, L);
+      const double dy = (q.y - y) + image_at(mq, 16, L);
+      const double dz = (q.z - z) + image_at(mq, 18, L);
       const double dist2 = dx * dx + dy * dy + dz * dz;
       if (dist2 < rc2 && !((mq >> 20) != 0u && q.w == selfw)) {
         if (DEPTH == 0) {
@@ -1175,8 +1179,8 @@ __device__ inline void scan_chunks(const RepDev& R, int n, double cs, double sig
           hi2[r] = rrow[wlo ? z1 + 1 : z1 - n + 1];
         }
       }
-      code[r] = (image_code(image_shift(hw[1] + dj, n, L)) << 4) | (whi ? 8u : 0u) |
-                (wlo ? 1u : 0u);
+      code[r] = (image_code(image_shift(hw[1] + dj, n, L)) << 4) | (whi ? 4u : 0u) |
+                (wlo ? 3u : 0u);
     }
     // 2. the slab's non-empty ranges in visiting order
     int nseg = 0;
@@ -1216,8 +1220,8 @@ __device__ inline void scan_chunks(const RepDev& R, int n, double cs, double sig
           st = (int)nst;
           mc = nmc;
           rem = (int)(mc & 0xffffu);
-          sy = image_of((mc >> 16) & 3u, L);
-          sz = image_of((mc >> 18) & 3u, L);
+          sy = image_at(mc, 16, L);
+          sz = image_at(mc, 18, L);
           k++;
           if (k < nseg) {
             nst = seg[(2 * k) * kBlk];
@@ -1430,6 +1434,296 @@ __global__ void __launch_bounds__(kBlk, (UPD && WALK == 1 && DEPTH <= 16) ? 4
                                                &mv);
 }
 
+// ---------------------------------------------------------------------------
+// Flat slab walk (BD_WALK_FLAT): the slab walk's ranges, partners and order,
+// with the candidates of a wavefront's 64 particles tested together.
+//
+// Per stencil slab (pass, di), every lane lists its particle's ranges (the
+// slab walk's rows, gap tests and z ranges) in its LDS column, one word per
+// range: slot | y image code << 22 | z image code << 24 | length << 26, the
+// slot counted from the replica's first (big-grid) slot.
+// A wavefront prefix sum of the lanes' candidate counts places every
+// particle's candidates, in its own visiting order and after those of the
+// lanes before it, in one LDS array (owner lane in the top six bits), up to
+// kFlatCap at a time; the 64 lanes then test 64 candidates per trip, whoever
+// owns them, so no lane idles while another still has candidates.  A partner
+// inside the cutoff goes to its owner's record column (global memory, the
+// deferral record of the slab walk: slot | image codes | pass bit) at the
+// owner's running count plus its rank among the owner's partners in the
+// trip; an owner's candidates are contiguous and in visiting order, so the
+// records come out in the reference order.  A candidate's position is
+// gathered from its staged entry alone, so the entries and gathers of
+// kFlatBatch trips are issued together before the first is tested.  After
+// the last slab every lane evaluates its particle's records in that order
+// with the reference operations (the deferral drain, four records loaded at
+// a time), or, when more partners were found than the column holds, walks
+// its stencil again with direct evaluation.  Needs replicas of fewer than
+// 2^22 slots (both grids) and ranges of at most 63 slots (ntags <= 12).
+// ---------------------------------------------------------------------------
+constexpr int kFlatCap = 1024;              // staged candidates per round
+constexpr int kFlatDepth = 32;              // record column per particle (maximum depth)
+constexpr unsigned int kFlatSlotBits = 22;  // grid slot bits of a staged entry
+constexpr unsigned int kFlatSlotMask = (1u << kFlatSlotBits) - 1u;
+constexpr int kFlatMaxTags = 12;            // 5 cells x 12 slots fit the 6-bit length
+constexpr int kFlatBatch = 4;               // trips whose gathers are issued together
+
+// Inclusive prefix sum over the 64 lanes of a wavefront: within each row of
+// 16 lanes by shifts of 1, 2, 4 and 8 lanes (lanes without a source add 0),
+// then the last lane of row 0 (and of row 2) added to row 1 (and row 3), and
+// the last lane of row 1 to rows 2 and 3.
+__device__ inline int wave_prefix_sum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
+// The ranges of one slab of the flat walk (rows dj = -NRW..NRW, those beyond
+// the particle's own nr or the cutoff skipped), in the slab walk's visiting
+// order, with every row's bounds loaded before any range is written.
+template <int NRW>
+__device__ inline void flat_list(int nr, double gx2, double lim, const double* g2lo,
+                                 const double* g2hi, double uy, double cs, int n, const int hw[3],
+                                 int a, const int* __restrict__ rs_g, int goff, double L,
+                                 unsigned int* myseg, int& nseg, int& tot) {
+  constexpr int W = 2 * NRW + 1;
+  int lo[W], hi[W], lo2[W], hi2[W];
+  unsigned int code[W];  // y image << 4 | z image of range b << 2 | z image of range a
+#pragma unroll
+  for (int r = 0; r < W; r++) {
+    const int dj = r - NRW;
+    const double gy = slab_gap(dj, cs, uy);
+    const double gxy2 = gx2 + gy * gy;
+    int b = hw[1] + dj;
+    b = b < 0 ? b + n : (b >= n ? b - n : b);
+    int klo = 0, khi = 0;
+#pragma unroll
+    for (int k = 0; k < NRW; k++) {
+      if (k < nr && gxy2 + g2lo[k] <= lim) klo = -(k + 1);
+      if (k < nr && gxy2 + g2hi[k] <= lim) khi = k + 1;
+    }
+    const int* rrow = rs_g + (a + n * b) * (n + 1);
+    const int z0 = hw[2] + klo, z1 = hw[2] + khi;
+    const bool wlo = z0 < 0, whi = z1 >= n;
+    lo[r] = hi[r] = lo2[r] = hi2[r] = 0;
+    if (dj >= -nr && dj <= nr && gxy2 <= lim) {
+      lo[r] = rrow[wlo ? z0 + n : z0];
+      hi[r] = rrow[(wlo || whi) ? n : z1 + 1];
+      if (wlo || whi) {
+        lo2[r] = rrow[0];
+        hi2[r] = rrow[wlo ? z1 + 1 : z1 - n + 1];
+      }
+    }
+    code[r] = (image_code(image_shift(hw[1] + dj, n, L)) << 4) | (whi ? 4u : 0u) |
+              (wlo ? 3u : 0u);
+  }
+#pragma unroll
+  for (int r = 0; r < W; r++) {
+    const unsigned int yc = (code[r] >> 4) << 22;
+    if (hi[r] > lo[r]) {
+      myseg[nseg * kBlk] = (unsigned int)(lo[r] + goff) | yc | ((code[r] & 3u) << 24) |
+                           ((unsigned int)(hi[r] - lo[r]) << 26);
+      tot += hi[r] - lo[r];
+      nseg++;
+    }
+    if (hi2[r] > lo2[r]) {
+      myseg[nseg * kBlk] = (unsigned int)(lo2[r] + goff) | yc | (((code[r] >> 2) & 3u) << 24) |
+                           ((unsigned int)(hi2[r] - lo2[r]) << 26);
+      tot += hi2[r] - lo2[r];
+      nseg++;
+    }
+  }
+}
+
+template <int POT>
+__global__ void __launch_bounds__(kBlk, 4)
+    k_forces_flat(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
+                  const double* __restrict__ px, const double* __restrict__ py,
+                  const double* __restrict__ pz, const int* __restrict__ rs,
+                  const double4* __restrict__ cpos, double* __restrict__ ffx,
+                  double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
+                  double* __restrict__ pot, unsigned int* __restrict__ recs, int rstride,
+                  int depth) {
+  __shared__ unsigned int seg[kSlabSegs * kBlk];
+  __shared__ unsigned int cand[kFlatCap];
+  __shared__ double4 opos[kBlk];  // owner x, y, z and the slab's x shift
+  __shared__ int4 oinf[kBlk];     // species, x code | pass bit, self index, first flat index
+  __shared__ int ond[kBlk];       // partners recorded so far
+  const int4 t = tbl[blockIdx.x];
+  if (t.x < 0) return;
+  const int lane = (int)threadIdx.x;
+  const int i = t.y + lane;
+  const bool live = i < t.z;
+  const RepDev& R = reps[t.x];
+  const int p = R.p_off + (live ? i : t.y);
+  const double x = px[p], y = py[p], z = pz[p];
+  double fx = 0.0, fy = 0.0, fz = 0.0, v = 0.0, u = 0.0;
+  const int g = i < R.nb ? 0 : 1;
+  if (!R.fast) {  // grids too narrow for the slab walks (one replica per workgroup)
+    if (live) {
+      forces_global<POT, false>(R, i, x, y, z, rs, cpos, fx, fy, fz, v, u);
+      ffx[p] = fx;
+      ffy[p] = fy;
+      ffz[p] = fz;
+      vir[p] = v;
+      pot[p] = u;
+    }
+    return;
+  }
+  const double L = R.side;
+  ond[lane] = 0;
+  opos[lane] = make_double4(x, y, z, 0.0);
+  unsigned int* myseg = seg + lane;
+  unsigned int* rbase = recs + R.p_off + t.y;  // record column of lane 0's particle
+  const double4* cbase = cpos + R.tag_off[0];  // the replica's slots
+  const double rc2b = R.rc2_same[0], rc2s = R.rc2_same[1];
+  for (int pass = 0; pass < 2; pass++) {
+    const int gg = pass == 0 ? g : 1 - g;
+    const int nr = pass == 0 ? R.nrange_same[g] : R.nrange_cross[g];
+    const double rc2 = pass == 0 ? R.rc2_same[g] : R.rc2_bs;
+    const double lim = rc2 + 1e-9;
+    int hw[3] = {0, 0, 0};
+    const bool ok = live && cell_coords(R, gg, x, y, z, hw[0], hw[1], hw[2]);
+    const int n = R.ncell[gg];
+    const double cs = R.cell_side[gg];
+    const int* rs_g = rs + R.rs_off[gg];
+    const int goff = R.tag_off[gg] - R.tag_off[0];  // the grid's first slot in the replica
+    const double ux = x - (double)hw[0] * cs, uy = y - (double)hw[1] * cs,
+                 uz = z - (double)hw[2] * cs;
+    double g2lo[2], g2hi[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const double lo = slab_gap(-(k + 1), cs, uz);
+      const double hi = slab_gap(k + 1, cs, uz);
+      g2lo[k] = lo * lo;
+      g2hi[k] = hi * hi;
+    }
+    const int nrw = __any(ok && nr > 1) ? 2 : 1;
+    const unsigned int pb = pass == 0 ? 0u : kPassBit;
+    for (int di = -nrw; di <= nrw; di++) {
+      // 1. this particle's ranges of the slab, in visiting order
+      int nseg = 0, tot = 0;
+      double shx = 0.0;
+      if (ok && di >= -nr && di <= nr) {
+        const double gx = slab_gap(di, cs, ux);
+        const double gx2 = gx * gx;
+        shx = image_shift(hw[0] + di, n, L);
+        if (gx2 <= lim) {
+          int a = hw[0] + di;
+          a = a < 0 ? a + n : (a >= n ? a - n : a);
+          if (nrw == 1)
+            flat_list<1>(nr, gx2, lim, g2lo, g2hi, uy, cs, n, hw, a, rs_g, goff, L, myseg, nseg,
+                         tot);
+          else
+            flat_list<2>(nr, gx2, lim, g2lo, g2hi, uy, cs, n, hw, a, rs_g, goff, L, myseg, nseg,
+                         tot);
+        }
+      }
+      // 2. the lanes' candidates one after the other: exclusive prefix sum
+      const int incl = wave_prefix_sum(tot);
+      const int T = __builtin_amdgcn_readlane(incl, 63);
+      const int base = incl - tot;
+      opos[lane].w = shx;
+      oinf[lane] = make_int4(g, (int)((image_code(shx) << kSlotBits) | pb), pass == 0 ? i : -1,
+                             base);
+      for (int w0 = 0; w0 < T; w0 += kFlatCap) {
+        // 3. stage the candidates with flat index in [w0, w0 + kFlatCap)
+        int f = base;
+        for (int k = 0; k < nseg; k++) {
+          const unsigned int sw = myseg[k * kBlk];
+          const int len = (int)(sw >> 26);
+          const int lo = max(f, w0), hi = min(f + len, w0 + kFlatCap);
+          const unsigned int e0 = (sw & 0x3ffffffu) | ((unsigned int)lane << 26);
+          for (int ff = lo; ff < hi; ff++) cand[ff - w0] = e0 + (unsigned int)(ff - f);
+          f += len;
+        }
+        __syncthreads();
+        // 4. 64 candidates per trip, kFlatBatch trips at a time: their entries
+        //    and position gathers are all issued before the first is tested
+        const int m = min(kFlatCap, T - w0);
+        for (int it0 = 0; it0 < m; it0 += 64 * kFlatBatch) {
+          unsigned int eb[kFlatBatch];
+          double4 qb[kFlatBatch];
+#pragma unroll
+          for (int k = 0; k < kFlatBatch; k++) {
+            const int e = it0 + 64 * k + lane;
+            eb[k] = cand[e < m ? e : 0];
+          }
+#pragma unroll
+          for (int k = 0; k < kFlatBatch; k++) qb[k] = cbase[eb[k] & kFlatSlotMask];
+#pragma unroll
+          for (int k = 0; k < kFlatBatch; k++) {
+            const int it = it0 + 64 * k;
+            const unsigned int ent = eb[k];
+            const double4 q = qb[k];
+            const bool valid = it + lane < m;
+            const int o = (int)(ent >> 26);
+            const double4 op = opos[o];
+            const int4 oi = oinf[o];
+            const double dx = (q.x - op.x) + op.w;
+            const double dy = (q.y - op.y) + image_at(ent, 22, L);
+            const double dz = (q.z - op.z) + image_at(ent, 24, L);
+            const double dist2 = dx * dx + dy * dy + dz * dz;
+            const double orc2 = pass == 0 ? (oi.x == 0 ? rc2b : rc2s) : R.rc2_bs;
+            const bool hit = valid && dist2 < orc2 && q.w != (double)oi.z;
+            const unsigned long long hm = __ballot(hit);
+            if (hm != 0ull) {
+              // partners below this lane in the trip, and below the owner's first lane
+              const int mb = (int)__builtin_amdgcn_mbcnt_hi(
+                  (unsigned int)(hm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)hm, 0u));
+              const int fo = max(0, oi.w - w0 - it);
+              const int before = __shfl(mb, fo, 64);
+              if (hit) {
+                const int pos = ond[o] + mb - before;
+                if (pos < depth)
+                  rbase[(unsigned int)pos * (unsigned int)rstride + (unsigned int)o] =
+                      (ent & kFlatSlotMask) | (((ent >> 22) & 15u) << 27) | (unsigned int)oi.y;
+                atomicAdd(&ond[o], 1);
+              }
+            }
+          }
+        }
+        __syncthreads();  // the next round restages cand
+      }
+    }
+  }
+  __syncthreads();  // the records of the other lanes are visible
+  if (!live) return;
+  const int nd = ond[lane];
+  if (nd > depth) {
+    // more partners than the column holds: the direct walk, same order
+    forces_global<POT, false>(R, i, x, y, z, rs, cpos, fx, fy, fz, v, u);
+  } else {
+    const double s0 = R.sigma[g], s1 = R.sigma_bs;
+    for (int e0 = 0; e0 < nd; e0 += 4) {
+      unsigned int wc[4];
+      double4 qc[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) wc[k] = recs[(size_t)(e0 + k < nd ? e0 + k : e0) * rstride + p];
+#pragma unroll
+      for (int k = 0; k < 4; k++) qc[k] = cbase[wc[k] & kSlotMask];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (e0 + k < nd) {
+          const double dx = (qc[k].x - x) + image_at(wc[k], kSlotBits, L);
+          const double dy = (qc[k].y - y) + image_at(wc[k], kSlotBits + 2, L);
+          const double dz = (qc[k].z - z) + image_at(wc[k], kSlotBits + 4, L);
+          const double dist2 = dx * dx + dy * dy + dz * dz;
+          pair_add<POT>(R, (wc[k] & kPassBit) ? s1 : s0, dist2, dx, dy, dz, fx, fy, fz, v, u);
+        }
+      }
+    }
+  }
+  ffx[p] = fx;
+  ffy[p] = fy;
+  ffz[p] = fz;
+  vir[p] = v;
+  pot[p] = u;
+}
+
 // One stencil pass (0: same species, 1: the other species, main.cu:576-585)
 // of replica-local particle i with a slab walk (WALK 1..3), on grids of at
 // least 2 nrange + 3 cells.  part selects the slabs di: -1 all, 0 = [-nrange,
@@ -1483,11 +1777,9 @@ __device__ inline void pair_terms_q(const RepDev& R, double x, double y, double 
                                     unsigned int wc, const double4& qc, double& qx, double& qy,
                                     double& qz, double& wv, double& e) {
   const double L = R.side;
-  const unsigned int cx = (wc >> kSlotBits) & 3u, cy = (wc >> (kSlotBits + 2)) & 3u,
-                     cz = (wc >> (kSlotBits + 4)) & 3u;
-  const double dx = (qc.x - x) + (cx == 0u ? 0.0 : (cx == 1u ? -L : L));
-  const double dy = (qc.y - y) + (cy == 0u ? 0.0 : (cy == 1u ? -L : L));
-  const double dz = (qc.z - z) + (cz == 0u ? 0.0 : (cz == 1u ? -L : L));
+  const double dx = (qc.x - x) + image_at(wc, kSlotBits, L);
+  const double dy = (qc.y - y) + image_at(wc, kSlotBits + 2, L);
+  const double dz = (qc.z - z) + image_at(wc, kSlotBits + 4, L);
   const double dist2 = dx * dx + dy * dy + dz * dz;
   const double dist_inv = sqrt(1.0 / dist2);
   double nf;
@@ -2806,6 +3098,11 @@ struct bd_sys {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int* list_ok = nullptr;  // [R]
   bool defer_ok = true;  // every grid's slot index fits the 26-bit record
+  // flat walk: every grid below 2^22 slots with at most kFlatMaxTags per
+  // cell; its partner records (kFlatDepth per particle, allocated on first use)
+  bool flat_ok = true;
+  unsigned int* frec = nullptr;
+  int flat_depth = 32;  // records used per particle (kFlatDepth at most)
   std::vector<int4> tbl_split_h;
   int4* tbl_split = nullptr;
   // waiting-time origins: BD_MAX_ORIGINS slots of raw positions (x, y, z)
@@ -2931,6 +3228,14 @@ int list_invalidate(bd_sys* s) {
   HIP_TRY(hipMemcpyAsync(s->lflag, ones.data(), sizeof(int) * ones.size(), hipMemcpyHostToDevice,
                          s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
+  return BD_OK;
+}
+
+// The flat walk's partner records: kFlatDepth per particle, allocated on first use.
+int flat_setup(bd_sys* s) {
+  if (!s->flat_ok)
+    return fail(BD_E_ARG, "the flat walk needs replicas below 2^22 slots and ntags <= 12");
+  if (!s->frec) return dev_alloc(&s->frec, (size_t)kFlatDepth * s->n_total);
   return BD_OK;
 }
 
@@ -3112,6 +3417,27 @@ void launch_forces_phs_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const in
 
 bool phs_kind(int pot) { return pot == BD_POT_WCA49_50 || pot == BD_POT_WCA49_50_AO; }
 
+// The flat walk (bd_set_walk has allocated its records).
+void launch_flat(bd_sys* s, const int4* tbl, int nblk) {
+  switch (s->pot_kind) {
+    case BD_POT_WCA:
+      launch(s, k_forces_flat<BD_POT_WCA>, nblk, kBlk, s->reps_d, tbl, s->px, s->py, s->pz, s->rs,
+             s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->frec, s->n_total,
+             s->flat_depth);
+      break;
+    case BD_POT_WCA_MODIFIED:
+      launch(s, k_forces_flat<BD_POT_WCA_MODIFIED>, nblk, kBlk, s->reps_d, tbl, s->px, s->py,
+             s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->frec, s->n_total,
+             s->flat_depth);
+      break;
+    default:
+      launch(s, k_forces_flat<BD_POT_HARMONIC>, nblk, kBlk, s->reps_d, tbl, s->px, s->py, s->pz,
+             s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->frec, s->n_total,
+             s->flat_depth);
+      break;
+  }
+}
+
 template <bool NEIGH>
 void launch_forces_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* tbl, int nblk) {
   if (s->pot_kind == BD_POT_WCA49_50)
@@ -3123,6 +3449,8 @@ void launch_forces_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* 
     launch_forces_k<true, 0, 0, false>(s, ncounts, nlist, maxn, tbl, nblk);
   else if (s->walk == BD_WALK_LIST)
     launch_list(s, tbl, nblk);
+  else if (s->walk == BD_WALK_FLAT)
+    launch_flat(s, tbl, nblk);
   else if (s->walk >= 1)
     launch_slab_forces<false>(s, tbl, nblk);
   else
@@ -3182,7 +3510,7 @@ void launch_split_t(bd_sys* s) {
 // 3.02e9 on 256 replicas).
 bool fusable(const bd_sys* s) {
   if (s->fuse == 0 || s->order != BD_ORDER_REFERENCE || s->brick != 0 || s->walk < 1 ||
-      s->walk == BD_WALK_LIST || phs_kind(s->pot_kind))
+      s->walk == BD_WALK_LIST || s->walk == BD_WALK_FLAT || phs_kind(s->pot_kind))
     return false;
   if (s->fuse == 2 && (s->pass_split == 0 || s->defer == 0)) return false;
   for (int r = 0; r < s->R; r++)
@@ -3423,6 +3751,7 @@ void free_all(bd_sys* s) {
   if (s->tbl_split) (void)hipFree(s->tbl_split);
   if (s->origins) (void)hipFree(s->origins);
   if (s->callbuf) (void)hipFree(s->callbuf);
+  if (s->frec) (void)hipFree(s->frec);
   void* lists[] = {s->nlist, s->ncount, s->rbx, s->rby, s->rbz, s->lflag, s->list_ok, s->pos4};
   for (void* p : lists)
     if (p) (void)hipFree(p);
@@ -3723,6 +4052,14 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
       if ((int64_t)R.ncell[k] * R.ncell[k] * R.ncell[k] * R.ntags[k] > (int64_t)kSlotMask)
         s->defer_ok = false;
   if (!s->defer_ok) s->defer = 0;
+  for (const RepDev& R : s->reps) {
+    int64_t slots = 0;
+    for (int k = 0; k < 2; k++) {
+      slots += (int64_t)R.ncell[k] * R.ncell[k] * R.ncell[k] * R.ntags[k];
+      if (R.ntags[k] > kFlatMaxTags) s->flat_ok = false;
+    }
+    if (slots > (int64_t)kFlatSlotMask) s->flat_ok = false;
+  }
   s->tags_total = (size_t)tag_off;
   int rc = BD_OK;
 #define ALLOC(ptr, n)                 \
@@ -4298,7 +4635,7 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
     if (rc_ != BD_OK) return rc_;
   }
   // kernel shapes: the row, slab and chunked slab walks at three deferral
-  // depths, the LDS bricks.  The list walk is not among them: forces-only
+  // depths, the LDS bricks, the flat walk.  The list walk is not among them: forces-only
   // launches between rebuilds would not see its rebuild cost, which decides
   // it (measured slower on the workloads here).
   struct Shape {
@@ -4314,7 +4651,7 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
       {0, 16, BD_WALK_SLABS, 2},  {0, 32, BD_WALK_SLABS, 2},  {0, 16, BD_WALK_CHUNK2, 2},
       {0, 32, BD_WALK_CHUNK2, 2}, {0, 16, BD_WALK_CHUNK4, 2}, {0, 32, BD_WALK_CHUNK4, 2},
       {1, 0, BD_WALK_ROWS, 0},    {2, 0, BD_WALK_ROWS, 0},    {3, 0, BD_WALK_ROWS, 0},
-      {4, 0, BD_WALK_ROWS, 0}};
+      {4, 0, BD_WALK_ROWS, 0},    {0, 0, BD_WALK_FLAT, 0}};
   Shape best{0, 0, BD_WALK_ROWS, 0};
   double best_ms = 1e300;
   hipEvent_t e0, e1;
@@ -4326,6 +4663,7 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
                                            c.walk != keep.walk || c.split != keep.split))
       continue;
     if (c.defer != 0 && !s->defer_ok) continue;
+    if (c.walk == BD_WALK_FLAT && (phs_kind(s->pot_kind) || flat_setup(s) != BD_OK)) continue;
     // the 49-50 potentials run one kernel whatever the shape: time it once
     if (phs_kind(s->pot_kind) && (c.brick != 0 || c.defer != 16 || c.walk != BD_WALK_SLABS ||
                                   c.split != 0))
@@ -4371,11 +4709,26 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
   return BD_OK;
 }
 
+int bd_set_flat_depth(bd_sys* s, int32_t depth) {
+  if (!s) return fail(BD_E_ARG, "null handle");
+  if (depth < 1 || depth > kFlatDepth) return fail(BD_E_ARG, "depth must be 1..32");
+  {
+    const int rc_ = drop_graphs(s);
+    if (rc_ != BD_OK) return rc_;
+  }
+  s->flat_depth = depth;
+  return BD_OK;
+}
+
 int bd_set_walk(bd_sys* s, int32_t walk) {
   if (!s) return fail(BD_E_ARG, "null handle");
   if (walk != BD_WALK_ROWS && walk != BD_WALK_SLABS && walk != BD_WALK_CHUNK2 &&
-      walk != BD_WALK_CHUNK4 && walk != BD_WALK_LIST)
-    return fail(BD_E_ARG, "walk must be BD_WALK_ROWS, _SLABS, _CHUNK2, _CHUNK4 or _LIST");
+      walk != BD_WALK_CHUNK4 && walk != BD_WALK_LIST && walk != BD_WALK_FLAT)
+    return fail(BD_E_ARG, "walk must be BD_WALK_ROWS, _SLABS, _CHUNK2, _CHUNK4, _LIST or _FLAT");
+  if (walk == BD_WALK_FLAT) {
+    const int rc = flat_setup(s);
+    if (rc != BD_OK) return rc;
+  }
   {
     const int rc_ = drop_graphs(s);
     if (rc_ != BD_OK) return rc_;

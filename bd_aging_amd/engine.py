@@ -233,9 +233,14 @@ class Engine:
         return v.value
 
     def set_walk(self, walk):
-        """Force walk: abi.BD_WALK_ROWS (0), _SLABS (1), _CHUNK2 (2), _CHUNK4 (3)
-        or _LIST (4, neighbour lists)."""
+        """Force walk: abi.BD_WALK_ROWS (0), _SLABS (1), _CHUNK2 (2), _CHUNK4 (3),
+        _LIST (4, neighbour lists) or _FLAT (5, a wavefront's candidates
+        tested together)."""
         check(self.lib.bd_set_walk(self.h, int(walk)))
+
+    def set_flat_depth(self, depth):
+        """Partner records per particle kept by the flat walk (1..32)."""
+        check(self.lib.bd_set_flat_depth(self.h, int(depth)))
 
     def walk(self):
         v = C.c_int32()

@@ -220,17 +220,31 @@ int bd_get_defer_depth(const bd_sys* s, int32_t* depth);
  * them as one stream (fewer idle lanes when range lengths differ between
  * particles); BD_WALK_CHUNK2 / _CHUNK4 walk the same stream with the range
  * bounds of a slab loaded together and the candidates loaded 2 / 4 at a
- * time (fewer exposed memory latencies).  Same pairs, order and arithmetic:
- * bit-identical. */
+ * time (fewer exposed memory latencies); BD_WALK_FLAT lists the ranges of
+ * a slab for the 64 particles of a wavefront, stages all their candidates
+ * in one LDS array and tests them 64 at a time whoever owns them, recording
+ * the partners inside the cutoff per particle in visiting order (a device
+ * buffer of 32 records per particle, allocated on first use; a particle
+ * with more partners walks its stencil again), then evaluates them in that
+ * order; it needs grids below 2^22 slots and ntags <= 12 (BD_E_ARG
+ * otherwise) and does not fuse the update.  Same pairs, order and
+ * arithmetic: bit-identical. */
 enum {
   BD_WALK_ROWS = 0,
   BD_WALK_SLABS = 1,
   BD_WALK_CHUNK2 = 2,
   BD_WALK_CHUNK4 = 3,
-  BD_WALK_LIST = 4
+  BD_WALK_LIST = 4,
+  BD_WALK_FLAT = 5
 };
 int bd_set_walk(bd_sys* s, int32_t walk);
 int bd_get_walk(const bd_sys* s, int32_t* walk);
+
+/* Records per particle that BD_WALK_FLAT keeps (1..32, default 32; no
+ * reference counterpart): a particle with more partners inside the cutoff
+ * walks its stencil again with direct evaluation, bit-identical.  Small
+ * values exercise that path.  BD_E_ARG outside 1..32. */
+int bd_set_flat_depth(bd_sys* s, int32_t depth);
 
 /* Neighbour lists (BD_WALK_LIST, no reference counterpart): each particle
  * lists every particle within rc + skin.  A move that takes some particle of
@@ -277,8 +291,9 @@ int bd_get_fused_update(const bd_sys* s, int32_t* on);
 
 /* Autotuner (no reference counterpart): measures the force kernel for every
  * shape (row, slab and chunked slab walks at deferral depths 0, 16 and 32,
- * the slab walks also with the pass split; bricks 1..4; all bit-identical by
- * construction) with forces-only launches on the current state, keeps the
+ * the slab walks also with the pass split; bricks 1..4; the flat walk where
+ * it applies; all bit-identical by construction) with forces-only launches on
+ * the current state, keeps the
  * fastest and returns its brick in *chosen (the walk, depth and split are
  * read with bd_get_walk / bd_get_defer_depth / bd_get_pass_split).  With
  * cache_path != NULL the choice is looked up in / appended to that text

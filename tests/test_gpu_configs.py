@@ -7,8 +7,9 @@
   binning's row-chunked path (rows longer than one wavefront).
 * The 256-replica uneven batch of the benchmark (N log-uniform in [1000,
   16000], phi in [0.52, 0.60], quench T in [0.05, 0.5]): the whole batch
-  stepped in one engine, eight replicas (smallest and largest N, extreme phi
-  and T) compared with the oracle bit for bit.
+  stepped in one engine (the slab walk, then the flat walk), eight replicas
+  (smallest and largest N, extreme phi and T) compared with the oracle bit
+  for bit.
 
 Reference: cell_locate.cu:155-209, get_forces.cu:297-558, update_ermak.cu:91-160,
 main.cu:213 (one seed per configuration) and 494-586 (the step loop).
@@ -62,7 +63,7 @@ def test_stress_config_cells_forces_and_steps(stress_positions):
     fo, vo, uo = o.forces()
     # default slab walk, the chunked four-wavefront split, the row walk
     for walk, depth, split in ((abi.BD_WALK_SLABS, 16, 0), (abi.BD_WALK_CHUNK4, 16, 2),
-                               (abi.BD_WALK_ROWS, 0, 0)):
+                               (abi.BD_WALK_ROWS, 0, 0), (abi.BD_WALK_FLAT, 0, 0)):
         e.set_walk(walk)
         e.set_defer_depth(depth)
         e.set_pass_split(split)
@@ -121,7 +122,9 @@ def test_256_replica_batch_against_oracle():
     e = Engine(params)
     for r, x in enumerate(xs):
         e.set_positions(x, r)
-    e.step(25)  # 5 steps at T = 1, the quench, 20 steps at T_r: one graph + 9 eager
+    e.step(12)  # 5 steps at T = 1, the quench, 7 steps at T_r (eager)
+    e.set_walk(abi.BD_WALK_FLAT)
+    e.step(13)  # 13 more under the flat walk (eager)
     picks = []
     for r in [int(np.argmin(n)), int(np.argmax(n)), int(np.argmin(phi)), int(np.argmax(phi)),
               int(np.argmin(T)), int(np.argmax(T)), 0, len(n) - 1, *range(1, 9)]:

@@ -390,6 +390,73 @@ def test_list_walk_batch_bit_identical():
             assert np.array_equal(a, b), r
 
 
+def test_flat_walk_bit_identical(config1_positions):
+    """The flat walk (a wavefront's candidates tested together, partners
+    recorded per particle) against the oracle: forces at every record depth
+    (depth 1 and 4 send most particles through the direct re-walk), then a
+    100-step trajectory identical to the oracle's."""
+    side, x = config1_positions
+    p = make_params(1000, 1000, side=side, T0=1.0, Tf=1.0, seed=17)
+    e, o = pair(p, x)
+    assert o.cell_locate() == 0
+    o.compute_forces()
+    fo = o.forces()
+    e.set_walk(abi.BD_WALK_FLAT)
+    assert e.walk() == abi.BD_WALK_FLAT
+    e.cell_locate()
+    for depth in (32, 1, 4, 12):
+        e.set_flat_depth(depth)
+        e.compute_forces()
+        assert all(np.array_equal(a, b) for a, b in zip(e.forces(), fo)), depth
+    with pytest.raises(BDError):
+        e.set_flat_depth(0)
+    with pytest.raises(BDError):
+        e.set_flat_depth(33)
+    e.set_flat_depth(32)
+    e.step(100)
+    assert o.step(100) == 0
+    assert np.array_equal(e.positions(), o.positions())
+    assert np.array_equal(e.raw_positions(), o.raw_positions())
+    assert all(np.array_equal(a, b) for a, b in zip(e.forces(), o.forces()))
+    e.close()
+    o.close()
+
+
+def test_flat_walk_batch_bit_identical():
+    """The flat walk on an uneven batch whose 64-particle blocks straddle the
+    species boundary (big counts not multiples of 64) and whose replicas need
+    one or two stencil rings, switched on and off during the run, against
+    the slab walk: every replica's state equal after 300 steps."""
+    specs = [(300, 340, 0.52, 1.5, 1), (701, 650, 0.55, 0.2, 2), (1000, 1000, 0.60, 0.05, 3),
+             (523, 480, 0.58, 0.7, 4)]
+    params, starts = [], []
+    for nb, ns, phi, T, seed in specs:
+        side = box_side(nb, ns, 1.4, 1.0, phi)
+        starts.append(relax_overlaps(nb, ns, side, seed=seed))
+        params.append(make_params(nb, ns, side=side, T0=1.0, Tf=T, t0=0.0, tf=0.0,
+                                  first_step=-5, seed=3000 + seed))
+    runs = []
+    for mode in ("slabs", "flat"):
+        e = Engine(params)
+        for r, x in enumerate(starts):
+            e.set_positions(x, r)
+        if mode == "flat":
+            e.set_walk(abi.BD_WALK_FLAT)
+        e.step(100)
+        if mode == "flat":
+            e.set_walk(abi.BD_WALK_SLABS)
+            e.step(37)
+            e.set_walk(abi.BD_WALK_FLAT)
+            e.step(163)
+        else:
+            e.step(200)
+        runs.append([[e.positions(r), e.raw_positions(r), *e.forces(r)] for r in range(len(specs))])
+        e.close()
+    for r in range(len(specs)):
+        for a, b in zip(runs[0][r], runs[1][r]):
+            assert np.array_equal(a, b), r
+
+
 @pytest.fixture(scope="module")
 def plain_positions():
     """N=32768 (16384 + 16384) at phi=0.58, no overlaps: the bench workload."""
@@ -414,7 +481,7 @@ def test_forces_bit_exact_at_bench_size(plain_positions):
               (0, 16, 3, 0), (0, 32, 3, 0), (0, 16, 1, 1), (0, 32, 1, 1), (0, 16, 2, 1),
               (0, 32, 2, 1), (0, 16, 3, 1), (0, 32, 3, 1), (0, 16, 1, 2), (0, 32, 1, 2),
               (0, 16, 2, 2), (0, 32, 2, 2), (0, 16, 3, 2), (0, 32, 3, 2), (0, 0, 4, 0),
-              (1, 16, 0, 0), (2, 16, 0, 0)]
+              (0, 0, 5, 0), (1, 16, 0, 0), (2, 16, 0, 0)]
     for b, depth, walk, split in shapes:
         e.set_brick(b)
         e.set_defer_depth(depth)
