@@ -1538,15 +1538,15 @@ __device__ inline void flat_list(int nr, double gx2, double lim, const double* g
   }
 }
 
-template <int POT>
+template <int POT, bool UPD>
 __global__ void __launch_bounds__(kBlk, 4)
     k_forces_flat(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
-                  const double* __restrict__ px, const double* __restrict__ py,
-                  const double* __restrict__ pz, const int* __restrict__ rs,
+                  double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+                  const int* __restrict__ rs,
                   const double4* __restrict__ cpos, double* __restrict__ ffx,
                   double* __restrict__ ffy, double* __restrict__ ffz, double* __restrict__ vir,
                   double* __restrict__ pot, unsigned int* __restrict__ recs, int rstride,
-                  int depth) {
+                  int depth, MoveArgs mv) {
   __shared__ unsigned int seg[kSlabSegs * kBlk];
   __shared__ unsigned int cand[kFlatCap];
   __shared__ double4 opos[kBlk];  // owner x, y, z and the slab's x shift
@@ -1570,6 +1570,11 @@ __global__ void __launch_bounds__(kBlk, 4)
       ffz[p] = fz;
       vir[p] = v;
       pot[p] = u;
+      if (UPD) {
+        double xi, yi, zi;
+        lane_normals(R, t.x, i, mv.ring, mv.lcgs, mv.draws, mv.total_lanes, xi, yi, zi);
+        fused_move(R, t.x, i, x, y, z, fx, fy, fz, xi, yi, zi, px, py, pz, mv);
+      }
     }
     return;
   }
@@ -1722,6 +1727,13 @@ __global__ void __launch_bounds__(kBlk, 4)
   ffz[p] = fz;
   vir[p] = v;
   pot[p] = u;
+  if (UPD) {
+    // the next step's move: the walks read the packed copy (cpos), never
+    // px/py/pz, and every lane of this workgroup read its position at the start
+    double xi, yi, zi;
+    lane_normals(R, t.x, i, mv.ring, mv.lcgs, mv.draws, mv.total_lanes, xi, yi, zi);
+    fused_move(R, t.x, i, x, y, z, fx, fy, fz, xi, yi, zi, px, py, pz, mv);
+  }
 }
 
 // One stencil pass (0: same species, 1: the other species, main.cu:576-585)
@@ -3102,6 +3114,9 @@ struct bd_sys {
   // cell; its partner records (kFlatDepth per particle, allocated on first use)
   bool flat_ok = true;
   unsigned int* frec = nullptr;
+  // the flat walk's block tables of 32 and 16 particles (pass split 1, 2)
+  std::vector<int4> tbl_flat_h[2];
+  int4* tbl_flat[2] = {nullptr, nullptr};
   int flat_depth = 32;  // records used per particle (kFlatDepth at most)
   std::vector<int4> tbl_split_h;
   int4* tbl_split = nullptr;
@@ -3417,23 +3432,32 @@ void launch_forces_phs_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const in
 
 bool phs_kind(int pot) { return pot == BD_POT_WCA49_50 || pot == BD_POT_WCA49_50_AO; }
 
-// The flat walk (bd_set_walk has allocated its records).
+// The flat walk (bd_set_walk has allocated its records); UPD: with the next
+// step's move.  Pass split 1 / 2 gives each wavefront 32 / 16 particles (all
+// 64 lanes still test the staged candidates): more wavefronts for systems too
+// small to fill the GPU.
+template <bool UPD>
 void launch_flat(bd_sys* s, const int4* tbl, int nblk) {
+  const MoveArgs mv = move_args(s);
+  if (s->pass_split > 0 && tbl == s->tbl_part) {  // 32 or 16 particles per wavefront
+    tbl = s->tbl_flat[s->pass_split - 1];
+    nblk = (int)s->tbl_flat_h[s->pass_split - 1].size();
+  }
   switch (s->pot_kind) {
     case BD_POT_WCA:
-      launch(s, k_forces_flat<BD_POT_WCA>, nblk, kBlk, s->reps_d, tbl, s->px, s->py, s->pz, s->rs,
-             s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->frec, s->n_total,
-             s->flat_depth);
+      launch(s, k_forces_flat<BD_POT_WCA, UPD>, nblk, kBlk, s->reps_d, tbl, s->px, s->py, s->pz,
+             s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->frec, s->n_total,
+             s->flat_depth, mv);
       break;
     case BD_POT_WCA_MODIFIED:
-      launch(s, k_forces_flat<BD_POT_WCA_MODIFIED>, nblk, kBlk, s->reps_d, tbl, s->px, s->py,
+      launch(s, k_forces_flat<BD_POT_WCA_MODIFIED, UPD>, nblk, kBlk, s->reps_d, tbl, s->px, s->py,
              s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->frec, s->n_total,
-             s->flat_depth);
+             s->flat_depth, mv);
       break;
     default:
-      launch(s, k_forces_flat<BD_POT_HARMONIC>, nblk, kBlk, s->reps_d, tbl, s->px, s->py, s->pz,
-             s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->frec, s->n_total,
-             s->flat_depth);
+      launch(s, k_forces_flat<BD_POT_HARMONIC, UPD>, nblk, kBlk, s->reps_d, tbl, s->px, s->py,
+             s->pz, s->rs, s->cpos, s->fx, s->fy, s->fz, s->vir, s->pot, s->frec, s->n_total,
+             s->flat_depth, mv);
       break;
   }
 }
@@ -3450,7 +3474,7 @@ void launch_forces_t(bd_sys* s, int* ncounts, int* nlist, int maxn, const int4* 
   else if (s->walk == BD_WALK_LIST)
     launch_list(s, tbl, nblk);
   else if (s->walk == BD_WALK_FLAT)
-    launch_flat(s, tbl, nblk);
+    launch_flat<false>(s, tbl, nblk);
   else if (s->walk >= 1)
     launch_slab_forces<false>(s, tbl, nblk);
   else
@@ -3510,9 +3534,10 @@ void launch_split_t(bd_sys* s) {
 // 3.02e9 on 256 replicas).
 bool fusable(const bd_sys* s) {
   if (s->fuse == 0 || s->order != BD_ORDER_REFERENCE || s->brick != 0 || s->walk < 1 ||
-      s->walk == BD_WALK_LIST || s->walk == BD_WALK_FLAT || phs_kind(s->pot_kind))
+      s->walk == BD_WALK_LIST || phs_kind(s->pot_kind))
     return false;
-  if (s->fuse == 2 && (s->pass_split == 0 || s->defer == 0)) return false;
+  if (s->fuse == 2 && s->walk != BD_WALK_FLAT && (s->pass_split == 0 || s->defer == 0))
+    return false;
   for (int r = 0; r < s->R; r++)
     if (s->reps[r].per_lane != 1) return false;
   return true;
@@ -3520,7 +3545,10 @@ bool fusable(const bd_sys* s) {
 
 // Forces of the current positions and the next step's update in one launch.
 void launch_forces_move(bd_sys* s) {
-  launch_slab_forces<true>(s, s->tbl_part, (int)s->tbl_part_h.size());
+  if (s->walk == BD_WALK_FLAT)
+    launch_flat<true>(s, s->tbl_part, (int)s->tbl_part_h.size());
+  else
+    launch_slab_forces<true>(s, s->tbl_part, (int)s->tbl_part_h.size());
 }
 
 void launch_forces(bd_sys* s) {
@@ -3752,6 +3780,8 @@ void free_all(bd_sys* s) {
   if (s->origins) (void)hipFree(s->origins);
   if (s->callbuf) (void)hipFree(s->callbuf);
   if (s->frec) (void)hipFree(s->frec);
+  for (int4* t : s->tbl_flat)
+    if (t) (void)hipFree(t);
   void* lists[] = {s->nlist, s->ncount, s->rbx, s->rby, s->rbz, s->lflag, s->list_ok, s->pos4};
   for (void* p : lists)
     if (p) (void)hipFree(p);
@@ -4030,12 +4060,15 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
     p_off += R.n;
     lane_off += R.lanes;
     add_blocks(s->tbl_part_h, r, R.n, 0);
+    add_blocks(s->tbl_flat_h[0], r, R.n, 0, 32);
+    add_blocks(s->tbl_flat_h[1], r, R.n, 0, 16);
     add_blocks(s->tbl_lane_h, r, R.lanes, 0);
     for (int k = 0; k < 2; k++)
       add_blocks(s->tbl_cell_h, r, R.ncell[k] * R.ncell[k], k,
                  kRowWaves * rows_per_wave(R.ncell[k]));
   }
   s->tbl_part_h = xcd_group(s->tbl_part_h);
+  for (auto& t : s->tbl_flat_h) t = xcd_group(t);
   s->tbl_lane_h = xcd_group(s->tbl_lane_h);
   s->tbl_cell_h = xcd_group(s->tbl_cell_h);
   if (p_off > INT32_MAX / 2 || tag_off > INT32_MAX || lane_off > INT32_MAX / 2 ||
@@ -4065,6 +4098,8 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
 #define ALLOC(ptr, n)                 \
   if (rc == BD_OK) rc = dev_alloc(&(ptr), (n));
   ALLOC(s->tbl_part, s->tbl_part_h.size());
+  ALLOC(s->tbl_flat[0], s->tbl_flat_h[0].size());
+  ALLOC(s->tbl_flat[1], s->tbl_flat_h[1].size());
   ALLOC(s->tbl_lane, s->tbl_lane_h.size());
   ALLOC(s->tbl_cell, s->tbl_cell_h.size());
   ALLOC(s->reps_d, (size_t)n_replicas);
@@ -4096,6 +4131,9 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
   if (e == hipSuccess)
     e = hipMemcpy(s->tbl_part, s->tbl_part_h.data(), sizeof(int4) * s->tbl_part_h.size(),
                   hipMemcpyHostToDevice);
+  for (int k = 0; k < 2 && e == hipSuccess; k++)
+    e = hipMemcpy(s->tbl_flat[k], s->tbl_flat_h[k].data(),
+                  sizeof(int4) * s->tbl_flat_h[k].size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(s->tbl_lane, s->tbl_lane_h.data(), sizeof(int4) * s->tbl_lane_h.size(),
                   hipMemcpyHostToDevice);
@@ -4651,7 +4689,8 @@ int bd_autotune(bd_sys* s, int32_t trial_launches, const char* cache_path, int32
       {0, 16, BD_WALK_SLABS, 2},  {0, 32, BD_WALK_SLABS, 2},  {0, 16, BD_WALK_CHUNK2, 2},
       {0, 32, BD_WALK_CHUNK2, 2}, {0, 16, BD_WALK_CHUNK4, 2}, {0, 32, BD_WALK_CHUNK4, 2},
       {1, 0, BD_WALK_ROWS, 0},    {2, 0, BD_WALK_ROWS, 0},    {3, 0, BD_WALK_ROWS, 0},
-      {4, 0, BD_WALK_ROWS, 0},    {0, 0, BD_WALK_FLAT, 0}};
+      {4, 0, BD_WALK_ROWS, 0},    {0, 0, BD_WALK_FLAT, 0},    {0, 0, BD_WALK_FLAT, 1},
+      {0, 0, BD_WALK_FLAT, 2}};
   Shape best{0, 0, BD_WALK_ROWS, 0};
   double best_ms = 1e300;
   hipEvent_t e0, e1;

@@ -226,9 +226,8 @@ int bd_get_defer_depth(const bd_sys* s, int32_t* depth);
  * the partners inside the cutoff per particle in visiting order (a device
  * buffer of 32 records per particle, allocated on first use; a particle
  * with more partners walks its stencil again), then evaluates them in that
- * order; it needs grids below 2^22 slots and ntags <= 12 (BD_E_ARG
- * otherwise) and does not fuse the update.  Same pairs, order and
- * arithmetic: bit-identical. */
+ * order; it needs replicas below 2^22 slots and ntags <= 12 (BD_E_ARG
+ * otherwise).  Same pairs, order and arithmetic: bit-identical. */
 enum {
   BD_WALK_ROWS = 0,
   BD_WALK_SLABS = 1,
@@ -272,7 +271,9 @@ int bd_get_list_skin(const bd_sys* s, double* skin);
  * evaluating the terms of their first partners into LDS for the first to
  * add in order.  More wavefronts for the same work, for systems too small to
  * fill the GPU.  Takes effect with a slab walk (BD_WALK_SLABS, _CHUNK2,
- * _CHUNK4) at a nonzero deferral depth.  Bit-identical.  BD_E_ARG for values
+ * _CHUNK4) at a nonzero deferral depth.  With BD_WALK_FLAT, 1 and 2 give
+ * each wavefront 32 and 16 particles instead of 64 (the 64 lanes still test
+ * the staged candidates together).  Bit-identical.  BD_E_ARG for values
  * other than 0, 1, 2. */
 int bd_set_pass_split(bd_sys* s, int32_t split);
 int bd_get_pass_split(const bd_sys* s, int32_t* split);
@@ -280,12 +281,13 @@ int bd_get_pass_split(const bd_sys* s, int32_t* split);
 /* Fused update (no reference counterpart): bd_step runs the update of each
  * step after the first (update_ermak.cu:91-160) inside the force kernel of
  * the step before, one launch fewer per step; same operations on the same
- * values, bit-identical.  It can apply to the thread-per-particle slab walks
- * (BD_WALK_SLABS, _CHUNK2, _CHUNK4) in reference order when every replica
- * has one particle per rannew64 lane (rng_lanes 0 or >= N).  mode 0: never;
- * 1: wherever it can; 2 (default): with a pass split at a nonzero deferral
- * depth only (where it is faster).  bd_get_fused_update reports whether it
- * applies with the current settings (0 or 1). */
+ * values, bit-identical.  It can apply to the slab walks (BD_WALK_SLABS,
+ * _CHUNK2, _CHUNK4, _FLAT) in reference order when every replica has one
+ * particle per rannew64 lane (rng_lanes 0 or >= N).  mode 0: never;
+ * 1: wherever it can; 2 (default): the flat walk, and the other slab walks
+ * with a pass split at a nonzero deferral depth (where it is faster).
+ * bd_get_fused_update reports whether it applies with the current settings
+ * (0 or 1). */
 int bd_set_fused_update(bd_sys* s, int32_t mode);
 int bd_get_fused_update(const bd_sys* s, int32_t* on);
 

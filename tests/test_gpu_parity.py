@@ -279,8 +279,12 @@ def test_pass_split_column_overflow(config1_positions):
     o.close()
 
 
-@pytest.mark.parametrize("split", [0, 1, 2])
-def test_fused_update_bit_identical(config1_positions, split):
+@pytest.mark.parametrize("walk,depth,split", [(abi.BD_WALK_CHUNK4, 16, 0),
+                                              (abi.BD_WALK_CHUNK4, 16, 1),
+                                              (abi.BD_WALK_CHUNK4, 16, 2),
+                                              (abi.BD_WALK_FLAT, 0, 0),
+                                              (abi.BD_WALK_FLAT, 0, 2)])
+def test_fused_update_bit_identical(config1_positions, walk, depth, split):
     """The update run inside the force kernel (default) against separate
     update launches: positions, raw positions, forces, rannew64 lanes and cell
     lists all equal after graph replays and eager steps, through a quench."""
@@ -289,8 +293,8 @@ def test_fused_update_bit_identical(config1_positions, split):
     runs = []
     for fused in (0, 1):
         e = Engine(p)
-        e.set_walk(abi.BD_WALK_CHUNK4)
-        e.set_defer_depth(16)
+        e.set_walk(walk)
+        e.set_defer_depth(depth)
         e.set_pass_split(split)
         e.set_fused_update(fused)
         assert e.fused_update() == fused
@@ -303,10 +307,11 @@ def test_fused_update_bit_identical(config1_positions, split):
     for a, b in zip(*runs):
         assert np.array_equal(np.asarray(a), np.asarray(b))
     e = Engine(p)
-    e.set_walk(abi.BD_WALK_CHUNK4)
-    e.set_defer_depth(16)
+    e.set_walk(walk)
+    e.set_defer_depth(depth)
     e.set_pass_split(split)
-    assert e.fused_update() == (1 if split else 0)  # default mode 2: with a pass split
+    # default mode 2: the flat walk, or a pass split
+    assert e.fused_update() == (1 if split or walk == abi.BD_WALK_FLAT else 0)
     e.close()
     e = Engine(make_params(1000, 1000, side=side, seed=11, rng_lanes=512))
     e.set_fused_update(1)
@@ -393,8 +398,10 @@ def test_list_walk_batch_bit_identical():
 def test_flat_walk_bit_identical(config1_positions):
     """The flat walk (a wavefront's candidates tested together, partners
     recorded per particle) against the oracle: forces at every record depth
-    (depth 1 and 4 send most particles through the direct re-walk), then a
-    100-step trajectory identical to the oracle's."""
+    (depth 1 and 4 send most particles through the direct re-walk) and at
+    64, 32 and 16 particles per wavefront, then a 100-step trajectory (16
+    per wavefront) identical to the oracle's, with the update fused into the
+    walk (graphs of fused pairs and eager steps)."""
     side, x = config1_positions
     p = make_params(1000, 1000, side=side, T0=1.0, Tf=1.0, seed=17)
     e, o = pair(p, x)
@@ -404,15 +411,16 @@ def test_flat_walk_bit_identical(config1_positions):
     e.set_walk(abi.BD_WALK_FLAT)
     assert e.walk() == abi.BD_WALK_FLAT
     e.cell_locate()
-    for depth in (32, 1, 4, 12):
+    for depth, split in ((32, 0), (1, 0), (4, 1), (12, 2), (32, 1), (32, 2)):
         e.set_flat_depth(depth)
+        e.set_pass_split(split)  # 64, 32 or 16 particles per wavefront
         e.compute_forces()
-        assert all(np.array_equal(a, b) for a, b in zip(e.forces(), fo)), depth
+        assert all(np.array_equal(a, b) for a, b in zip(e.forces(), fo)), (depth, split)
     with pytest.raises(BDError):
         e.set_flat_depth(0)
     with pytest.raises(BDError):
         e.set_flat_depth(33)
-    e.set_flat_depth(32)
+    assert e.fused_update()  # the flat walk moves the particles of the next step itself
     e.step(100)
     assert o.step(100) == 0
     assert np.array_equal(e.positions(), o.positions())
@@ -481,7 +489,7 @@ def test_forces_bit_exact_at_bench_size(plain_positions):
               (0, 16, 3, 0), (0, 32, 3, 0), (0, 16, 1, 1), (0, 32, 1, 1), (0, 16, 2, 1),
               (0, 32, 2, 1), (0, 16, 3, 1), (0, 32, 3, 1), (0, 16, 1, 2), (0, 32, 1, 2),
               (0, 16, 2, 2), (0, 32, 2, 2), (0, 16, 3, 2), (0, 32, 3, 2), (0, 0, 4, 0),
-              (0, 0, 5, 0), (1, 16, 0, 0), (2, 16, 0, 0)]
+              (0, 0, 5, 0), (0, 0, 5, 1), (0, 0, 5, 2), (1, 16, 0, 0), (2, 16, 0, 0)]
     for b, depth, walk, split in shapes:
         e.set_brick(b)
         e.set_defer_depth(depth)
