@@ -1460,7 +1460,7 @@ __global__ void __launch_bounds__(kBlk, (UPD && WALK == 1 && DEPTH <= 16) ? 4
 // its stencil again with direct evaluation.  Needs replicas of fewer than
 // 2^22 slots (both grids) and ranges of at most 63 slots (ntags <= 12).
 // ---------------------------------------------------------------------------
-constexpr int kFlatCap = 1024;              // staged candidates per round
+constexpr int kFlatCap = 960;               // staged candidates per round (15 trips)
 constexpr int kFlatDepth = 32;              // record column per particle (maximum depth)
 constexpr unsigned int kFlatSlotBits = 22;  // grid slot bits of a staged entry
 constexpr unsigned int kFlatSlotMask = (1u << kFlatSlotBits) - 1u;
@@ -1481,60 +1481,71 @@ __device__ inline int wave_prefix_sum(int v) {
   return v;
 }
 
+// Element idx of an array addressed by a 32-bit byte offset from its base
+// (one scalar base, one 32-bit vector offset per access); idx * sizeof(T)
+// must stay below 2^32.
+template <class T>
+__device__ inline T& at32(T* base, unsigned int idx) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (idx * (unsigned int)sizeof(T)));
+}
+template <class T>
+__device__ inline const T& at32(const T* base, unsigned int idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     (idx * (unsigned int)sizeof(T)));
+}
+
 // The ranges of one slab of the flat walk (rows dj = -NRW..NRW, those beyond
 // the particle's own nr or the cutoff skipped), in the slab walk's visiting
-// order, with every row's bounds loaded before any range is written.
+// order, with every row's bounds loaded before any range is written.  rs is
+// the engine's row-start table and rsa the first entry of row (a, 0) of this
+// grid in it (32-bit offsets from one base).  A range is written at the next
+// position whether or not it is empty; only a non-empty one advances nseg.
 template <int NRW>
 __device__ inline void flat_list(int nr, double gx2, double lim, const double* g2lo,
                                  const double* g2hi, double uy, double cs, int n, const int hw[3],
-                                 int a, const int* __restrict__ rs_g, int goff, double L,
+                                 int rsa, const int* __restrict__ rs, int goff,
                                  unsigned int* myseg, int& nseg, int& tot) {
   constexpr int W = 2 * NRW + 1;
   int lo[W], hi[W], lo2[W], hi2[W];
-  unsigned int code[W];  // y image << 4 | z image of range b << 2 | z image of range a
+  unsigned int code[W];  // y image << 22 | z image of range b << 2 | z image of range a
 #pragma unroll
   for (int r = 0; r < W; r++) {
     const int dj = r - NRW;
     const double gy = slab_gap(dj, cs, uy);
     const double gxy2 = gx2 + gy * gy;
-    int b = hw[1] + dj;
-    b = b < 0 ? b + n : (b >= n ? b - n : b);
+    const int hb = hw[1] + dj;
+    const int b = hb < 0 ? hb + n : (hb >= n ? hb - n : hb);
     int klo = 0, khi = 0;
 #pragma unroll
     for (int k = 0; k < NRW; k++) {
       if (k < nr && gxy2 + g2lo[k] <= lim) klo = -(k + 1);
       if (k < nr && gxy2 + g2hi[k] <= lim) khi = k + 1;
     }
-    const int* rrow = rs_g + (a + n * b) * (n + 1);
+    const unsigned int row = (unsigned int)(rsa + b * n * (n + 1));
     const int z0 = hw[2] + klo, z1 = hw[2] + khi;
     const bool wlo = z0 < 0, whi = z1 >= n;
     lo[r] = hi[r] = lo2[r] = hi2[r] = 0;
     if (dj >= -nr && dj <= nr && gxy2 <= lim) {
-      lo[r] = rrow[wlo ? z0 + n : z0];
-      hi[r] = rrow[(wlo || whi) ? n : z1 + 1];
+      lo[r] = at32(rs, row + (unsigned int)(wlo ? z0 + n : z0));
+      hi[r] = at32(rs, row + (unsigned int)((wlo || whi) ? n : z1 + 1));
       if (wlo || whi) {
-        lo2[r] = rrow[0];
-        hi2[r] = rrow[wlo ? z1 + 1 : z1 - n + 1];
+        lo2[r] = at32(rs, row);
+        hi2[r] = at32(rs, row + (unsigned int)(wlo ? z1 + 1 : z1 - n + 1));
       }
     }
-    code[r] = (image_code(image_shift(hw[1] + dj, n, L)) << 4) | (whi ? 4u : 0u) |
-              (wlo ? 3u : 0u);
+    code[r] = ((hb < 0 ? 3u : (hb >= n ? 1u : 0u)) << 22) | (whi ? 4u : 0u) | (wlo ? 3u : 0u);
   }
 #pragma unroll
   for (int r = 0; r < W; r++) {
-    const unsigned int yc = (code[r] >> 4) << 22;
-    if (hi[r] > lo[r]) {
-      myseg[nseg * kBlk] = (unsigned int)(lo[r] + goff) | yc | ((code[r] & 3u) << 24) |
-                           ((unsigned int)(hi[r] - lo[r]) << 26);
-      tot += hi[r] - lo[r];
-      nseg++;
-    }
-    if (hi2[r] > lo2[r]) {
-      myseg[nseg * kBlk] = (unsigned int)(lo2[r] + goff) | yc | (((code[r] >> 2) & 3u) << 24) |
-                           ((unsigned int)(hi2[r] - lo2[r]) << 26);
-      tot += hi2[r] - lo2[r];
-      nseg++;
-    }
+    const int la = hi[r] - lo[r], lb = hi2[r] - lo2[r];
+    const unsigned int yc = code[r] & (3u << 22);
+    myseg[nseg * kBlk] = (unsigned int)(lo[r] + goff) | yc | ((code[r] & 3u) << 24) |
+                         ((unsigned int)la << 26);
+    nseg += la > 0 ? 1 : 0;
+    myseg[nseg * kBlk] = (unsigned int)(lo2[r] + goff) | yc | (((code[r] >> 2) & 3u) << 24) |
+                         ((unsigned int)lb << 26);
+    nseg += lb > 0 ? 1 : 0;
+    tot += la + lb;
   }
 }
 
@@ -1550,7 +1561,8 @@ __global__ void __launch_bounds__(kBlk, 4)
   __shared__ unsigned int seg[kSlabSegs * kBlk];
   __shared__ unsigned int cand[kFlatCap];
   __shared__ double4 opos[kBlk];  // owner x, y, z and the slab's x shift
-  __shared__ int4 oinf[kBlk];     // species, x code | pass bit, self index, first flat index
+  __shared__ double2 orc[kBlk];   // owner cutoff^2 of the pass and self index (-1: none)
+  __shared__ int2 oinf[kBlk];     // owner x code | pass bit, first flat index
   __shared__ int ond[kBlk];       // partners recorded so far
   const int4 t = tbl[blockIdx.x];
   if (t.x < 0) return;
@@ -1584,7 +1596,6 @@ __global__ void __launch_bounds__(kBlk, 4)
   unsigned int* myseg = seg + lane;
   unsigned int* rbase = recs + R.p_off + t.y;  // record column of lane 0's particle
   const double4* cbase = cpos + R.tag_off[0];  // the replica's slots
-  const double rc2b = R.rc2_same[0], rc2s = R.rc2_same[1];
   for (int pass = 0; pass < 2; pass++) {
     const int gg = pass == 0 ? g : 1 - g;
     const int nr = pass == 0 ? R.nrange_same[g] : R.nrange_cross[g];
@@ -1594,7 +1605,6 @@ __global__ void __launch_bounds__(kBlk, 4)
     const bool ok = live && cell_coords(R, gg, x, y, z, hw[0], hw[1], hw[2]);
     const int n = R.ncell[gg];
     const double cs = R.cell_side[gg];
-    const int* rs_g = rs + R.rs_off[gg];
     const int goff = R.tag_off[gg] - R.tag_off[0];  // the grid's first slot in the replica
     const double ux = x - (double)hw[0] * cs, uy = y - (double)hw[1] * cs,
                  uz = z - (double)hw[2] * cs;
@@ -1608,6 +1618,7 @@ __global__ void __launch_bounds__(kBlk, 4)
     }
     const int nrw = __any(ok && nr > 1) ? 2 : 1;
     const unsigned int pb = pass == 0 ? 0u : kPassBit;
+    orc[lane] = make_double2(rc2, pass == 0 ? (double)i : -1.0);
     for (int di = -nrw; di <= nrw; di++) {
       // 1. this particle's ranges of the slab, in visiting order
       int nseg = 0, tot = 0;
@@ -1620,11 +1631,11 @@ __global__ void __launch_bounds__(kBlk, 4)
           int a = hw[0] + di;
           a = a < 0 ? a + n : (a >= n ? a - n : a);
           if (nrw == 1)
-            flat_list<1>(nr, gx2, lim, g2lo, g2hi, uy, cs, n, hw, a, rs_g, goff, L, myseg, nseg,
-                         tot);
+            flat_list<1>(nr, gx2, lim, g2lo, g2hi, uy, cs, n, hw, R.rs_off[gg] + a * (n + 1), rs,
+                         goff, myseg, nseg, tot);
           else
-            flat_list<2>(nr, gx2, lim, g2lo, g2hi, uy, cs, n, hw, a, rs_g, goff, L, myseg, nseg,
-                         tot);
+            flat_list<2>(nr, gx2, lim, g2lo, g2hi, uy, cs, n, hw, R.rs_off[gg] + a * (n + 1), rs,
+                         goff, myseg, nseg, tot);
         }
       }
       // 2. the lanes' candidates one after the other: exclusive prefix sum
@@ -1632,8 +1643,7 @@ __global__ void __launch_bounds__(kBlk, 4)
       const int T = __builtin_amdgcn_readlane(incl, 63);
       const int base = incl - tot;
       opos[lane].w = shx;
-      oinf[lane] = make_int4(g, (int)((image_code(shx) << kSlotBits) | pb), pass == 0 ? i : -1,
-                             base);
+      oinf[lane] = make_int2((int)((image_code(shx) << kSlotBits) | pb), base);
       for (int w0 = 0; w0 < T; w0 += kFlatCap) {
         // 3. stage the candidates with flat index in [w0, w0 + kFlatCap)
         int f = base;
@@ -1658,7 +1668,7 @@ __global__ void __launch_bounds__(kBlk, 4)
             eb[k] = cand[e < m ? e : 0];
           }
 #pragma unroll
-          for (int k = 0; k < kFlatBatch; k++) qb[k] = cbase[eb[k] & kFlatSlotMask];
+          for (int k = 0; k < kFlatBatch; k++) qb[k] = at32(cbase, eb[k] & kFlatSlotMask);
 #pragma unroll
           for (int k = 0; k < kFlatBatch; k++) {
             const int it = it0 + 64 * k;
@@ -1667,25 +1677,25 @@ __global__ void __launch_bounds__(kBlk, 4)
             const bool valid = it + lane < m;
             const int o = (int)(ent >> 26);
             const double4 op = opos[o];
-            const int4 oi = oinf[o];
+            const double2 oc = orc[o];
+            const int2 oi = oinf[o];
             const double dx = (q.x - op.x) + op.w;
             const double dy = (q.y - op.y) + image_at(ent, 22, L);
             const double dz = (q.z - op.z) + image_at(ent, 24, L);
             const double dist2 = dx * dx + dy * dy + dz * dz;
-            const double orc2 = pass == 0 ? (oi.x == 0 ? rc2b : rc2s) : R.rc2_bs;
-            const bool hit = valid && dist2 < orc2 && q.w != (double)oi.z;
+            const bool hit = valid && dist2 < oc.x && q.w != oc.y;
             const unsigned long long hm = __ballot(hit);
             if (hm != 0ull) {
               // partners below this lane in the trip, and below the owner's first lane
               const int mb = (int)__builtin_amdgcn_mbcnt_hi(
                   (unsigned int)(hm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)hm, 0u));
-              const int fo = max(0, oi.w - w0 - it);
-              const int before = __shfl(mb, fo, 64);
+              const int fo = max(0, oi.y - w0 - it);  // <= this lane for every valid lane
+              const int before = __builtin_amdgcn_ds_bpermute(fo << 2, mb);
               if (hit) {
                 const int pos = ond[o] + mb - before;
                 if (pos < depth)
-                  rbase[(unsigned int)pos * (unsigned int)rstride + (unsigned int)o] =
-                      (ent & kFlatSlotMask) | (((ent >> 22) & 15u) << 27) | (unsigned int)oi.y;
+                  at32(rbase, (unsigned int)pos * (unsigned int)rstride + (unsigned int)o) =
+                      (ent & kFlatSlotMask) | ((ent << 5) & 0x78000000u) | (unsigned int)oi.x;
                 atomicAdd(&ond[o], 1);
               }
             }
@@ -1707,9 +1717,11 @@ __global__ void __launch_bounds__(kBlk, 4)
       unsigned int wc[4];
       double4 qc[4];
 #pragma unroll
-      for (int k = 0; k < 4; k++) wc[k] = recs[(size_t)(e0 + k < nd ? e0 + k : e0) * rstride + p];
+      for (int k = 0; k < 4; k++)
+        wc[k] = at32(rbase, (unsigned int)(e0 + k < nd ? e0 + k : e0) * (unsigned int)rstride +
+                                (unsigned int)lane);
 #pragma unroll
-      for (int k = 0; k < 4; k++) qc[k] = cbase[wc[k] & kSlotMask];
+      for (int k = 0; k < 4; k++) qc[k] = at32(cbase, wc[k] & kSlotMask);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         if (e0 + k < nd) {
@@ -4093,6 +4105,10 @@ int bd_create(const bd_params* params, int32_t n_replicas, bd_sys** out) {
     }
     if (slots > (int64_t)kFlatSlotMask) s->flat_ok = false;
   }
+  // 32-bit byte offsets of the row starts and of the partner records
+  if ((int64_t)s->rs_total * 4 >= (1LL << 32) ||
+      (int64_t)s->n_total * kFlatDepth * 4 >= (1LL << 32))
+    s->flat_ok = false;
   s->tags_total = (size_t)tag_off;
   int rc = BD_OK;
 #define ALLOC(ptr, n)                 \

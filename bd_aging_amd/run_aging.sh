@@ -10,6 +10,7 @@
 # decade configuration dumps under configs/decadeK/.
 #
 #   usage: run_aging.sh SAMPLE_TAG     (run from a directory of your choice)
+#          BD_AGING_PARAMS=file run_aging.sh SAMPLE_TAG   (parameters overridden)
 set -e
 
 #########################Parameters#########################
@@ -45,6 +46,13 @@ samples_aging=10
 gr_nbins=400
 gr_range=15.0
 qmax=6.19
+
+# optional overrides: a file of name=value lines for the parameters above
+# (a shorter run, another density, ...)
+if [ -n "${BD_AGING_PARAMS:-}" ]; then
+  # shellcheck disable=SC1090
+  . "$BD_AGING_PARAMS"
+fi
 
 BIN="${BD_AGING_SELF:-$(cd "$(dirname "$0")" && pwd)/bd_aging_self}"
 

@@ -71,7 +71,8 @@ def pick_shape(eng, shape, trials):
         eng.set_walk(shape[2] if len(shape) > 2 else 0)
         eng.set_pass_split(shape[3] if len(shape) > 3 else 0)
         brick = shape[0]
-    return {"brick": brick, "walk": ["rows", "slabs", "chunks2", "chunks4", "list"][eng.walk()],
+    return {"brick": brick,
+            "walk": ["rows", "slabs", "chunks2", "chunks4", "list", "flat"][eng.walk()],
             "pass_split": eng.pass_split(), "fused_update": bool(eng.fused_update()),
             "defer_depth": eng.defer_depth(),
             "chosen_by": "autotuner" if shape is None else "--shape"}

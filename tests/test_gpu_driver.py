@@ -249,3 +249,34 @@ def test_system_mode_force_contraction_matches_oracle(tmp_path):
     en = np.loadtxt(sim / "results" / "energies_averaged.out")
     assert en.shape == (nsamples, 4)
     assert (sim / "results" / "gder_1.out").exists()
+
+
+def test_run_aging_script_end_to_end(tmp_path):
+    """run_aging.sh (the run-fself.sh workflow, run-fself.sh:1-197) with a short
+    parameter set: configurations made, one --aging run, and every decade's
+    F_s and MSD written with the reference's layout.  The numbers themselves
+    are checked against the oracle by the --aging test above; here F_s starts
+    below 1 and stays in [-1, 1], the MSD grows from a positive first value."""
+    script = os.path.join(ROOT, "bd_aging_amd", "run_aging.sh")
+    params = tmp_path / "short.params"
+    params.write_text("ngrain_big=200\nngrain_sml=200\nphi=0.45\nnumber_configs=2\n"
+                      "eq_steps_configs=50\ntranstime_aging=0.002\nruntime_aging=0.05\n"
+                      "initial_decade=1.0e-3\nnumber_initial_decades=2\ngr_nbins=50\n"
+                      "gr_range=5.0\n")
+    env = dict(os.environ, BD_AGING_PARAMS=str(params))
+    r = subprocess.run(["bash", script, "t1"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Simulation finished successfully" in r.stdout
+    run = [d for d in tmp_path.iterdir() if d.is_dir() and d.name.startswith("bigsml_200_200")]
+    assert len(run) == 1
+    res = run[0] / "simulation" / "results"
+    for k in (1, 2):
+        fs = np.loadtxt(res / f"decade{k}" / "fself.out", ndmin=2)
+        msd = np.loadtxt(res / f"decade{k}" / "msd_averaged.out", ndmin=2)
+        assert fs.shape[0] > 3 and msd.shape[0] == fs.shape[0]
+        assert np.all(np.diff(fs[:, 0]) > 0)  # lags increase
+        assert 0.0 < fs[0, 1] < 1.0 and np.all(np.abs(fs[:, 1]) <= 1.0)
+        assert msd[0, 1] > 0.0 and msd[-1, 1] > msd[0, 1]
+    assert (run[0] / "configs" / "decade0" / "init_config_2").exists()
+    assert (run[0] / "configs" / "decade2" / "init_config_1").exists()
