@@ -270,6 +270,38 @@ def evaluate(sur, X, Y, bd_seconds_per_sample):
             (pred_s + float(fallback.mean()) * bd_seconds_per_sample)}
 
 
+def baselines(Xtr, Ytr, Xte, Yte):
+    """RMS on the held-out points of two predictors that use the state
+    variables only: the mean training curve at the same waiting time, and a
+    least-squares cubic polynomial in (T, phi, log10 t_w) (all 20 monomials of
+    degree <= 3, one fit per lag)."""
+    ng = 3 * N_BINS
+    tw_tr, tw_te = np.round(Xtr[:, ng + 2], 9), np.round(Xte[:, ng + 2], 9)
+    mean_pred = np.empty_like(Yte)
+    for tw in np.unique(tw_te):
+        mean_pred[tw_te == tw] = Ytr[tw_tr == tw].mean(0)
+    mu, sd = Xtr[:, ng:].mean(0), Xtr[:, ng:].std(0)
+
+    def design(X):
+        v = (X[:, ng:] - mu) / sd
+        cols = [np.ones(len(X))]
+        for i in range(3):
+            cols.append(v[:, i])
+        for i in range(3):
+            for j in range(i, 3):
+                cols.append(v[:, i] * v[:, j])
+        for i in range(3):
+            for j in range(i, 3):
+                for k in range(j, 3):
+                    cols.append(v[:, i] * v[:, j] * v[:, k])
+        return np.stack(cols, 1)
+
+    coef, *_ = np.linalg.lstsq(design(Xtr), Ytr, rcond=None)
+    cubic_pred = design(Xte) @ coef
+    return {"rms_mean_curve_at_tw": float(np.sqrt(np.mean((mean_pred - Yte) ** 2))),
+            "rms_cubic_in_T_phi_logtw": float(np.sqrt(np.mean((cubic_pred - Yte) ** 2)))}
+
+
 def main(argv=None):
     import argparse
     import json
@@ -322,6 +354,7 @@ def main(argv=None):
     log(f"[surrogate] trained in {train_s:.1f} s, final losses {losses[-1]}")
     bd_per_sample = sum(walls[a.runs:]) / len(Xte)
     rep = evaluate(sur, Xte, Yte, bd_per_sample)
+    rep.update(baselines(Xtr, Ytr, Xte, Yte))
     rep.update({"train_samples": int(len(Xtr)), "epochs": a.epochs, "train_seconds": train_s,
                 "final_train_loss": losses[-1].tolist(), "lags": LAGS.tolist(), "k": K_FS,
                 "members": 5, "std_gate": sur.std_gate,

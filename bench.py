@@ -61,10 +61,59 @@ def batch_specs(R=256, seed=20251203):
     return n, phi, T
 
 
-def pick_shape(eng, shape, trials):
-    """Autotune the force kernel, or set the given (brick, deferral depth)."""
+def timed_steps(eng, steps):
+    """Seconds for `steps` BD steps replayed as graphs of steps - 1 (built
+    beforehand), as the timed runs replay them."""
+    G = eng.graph_steps()
+    eng.set_graph_steps(max(1, min(256, steps - 1)))
+    eng.prepare_step(steps)
+    t0 = time.perf_counter()
+    eng.step(steps)
+    dt = time.perf_counter() - t0
+    eng.set_graph_steps(G)
+    return dt
+
+
+def confirm_flat(eng, steps):
+    """The autotuner times forces-only launches, so it cannot see what fusing
+    the next step's move into the flat walk saves.  Time `steps` real steps
+    with the autotuned shape and with the flat walk (which fuses the move),
+    keep the faster.  Every shape gives the same bits, so the state advances
+    the same either way.  Returns the seconds of both (None: no flat walk)."""
+    tuned = (eng.brick(), eng.defer_depth(), eng.walk(), eng.pass_split())
+    if tuned[2] == 5:
+        return None
+    t_tuned = timed_steps(eng, steps)
+    try:
+        eng.set_brick(0)
+        eng.set_defer_depth(0)
+        eng.set_walk(5)
+        eng.set_pass_split(0)
+    except Exception:  # the flat walk does not apply to this engine
+        eng.set_brick(tuned[0])
+        eng.set_defer_depth(tuned[1])
+        eng.set_walk(tuned[2])
+        eng.set_pass_split(tuned[3])
+        return None
+    t_flat = timed_steps(eng, steps)
+    if t_flat >= t_tuned:
+        eng.set_brick(tuned[0])
+        eng.set_defer_depth(tuned[1])
+        eng.set_walk(tuned[2])
+        eng.set_pass_split(tuned[3])
+    return {"steps": steps, "autotuned_s": t_tuned, "flat_fused_s": t_flat}
+
+
+def pick_shape(eng, shape, trials, confirm_steps=0):
+    """Autotune the force kernel, or set the given (brick, deferral depth);
+    with confirm_steps, check the autotuner's pick against the fused flat walk
+    on real steps."""
+    confirm = None
     if shape is None:
         brick = eng.autotune(trials)
+        if confirm_steps > 0:
+            confirm = confirm_flat(eng, confirm_steps)
+            brick = eng.brick()
     else:
         eng.set_brick(shape[0])
         eng.set_defer_depth(shape[1])
@@ -75,7 +124,9 @@ def pick_shape(eng, shape, trials):
             "walk": ["rows", "slabs", "chunks2", "chunks4", "list", "flat"][eng.walk()],
             "pass_split": eng.pass_split(), "fused_update": bool(eng.fused_update()),
             "defer_depth": eng.defer_depth(),
-            "chosen_by": "autotuner" if shape is None else "--shape"}
+            "chosen_by": ("autotuner" if confirm is None else "autotuner, then timed steps against "
+                          "the fused flat walk") if shape is None else "--shape",
+            "confirm": confirm}
 
 
 def timed_launch_mode(eng, steps, graph_steps):
@@ -207,7 +258,7 @@ def run_batch(steps, warmup, graph_steps=16, shape=None):
     for r, x in enumerate(xs):
         eng.set_positions(x, r)
     eng.step(warmup)
-    shape = pick_shape(eng, shape, 10)
+    shape = pick_shape(eng, shape, 10, confirm_steps=100)
     log(f"[bench] batch force kernel {shape}")
     eng.mark_origin()
     mode = timed_launch_mode(eng, steps, graph_steps)
@@ -235,7 +286,7 @@ def run_stress(steps, warmup, graph_steps=16, shape=None):
     eng.set_graph_steps(graph_steps)
     eng.set_positions(x)
     eng.step(warmup)
-    shape = pick_shape(eng, shape, 10)
+    shape = pick_shape(eng, shape, 10, confirm_steps=200)
     timed_launch_mode(eng, steps, graph_steps)
     t0 = time.perf_counter()
     eng.step(steps)

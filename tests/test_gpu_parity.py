@@ -181,7 +181,7 @@ def test_every_brick_shape_bit_identical(config1_positions, pot):
     for bad in (-1, 8, 64):
         with pytest.raises(BDError):
             e.set_defer_depth(bad)
-    for bad in (-1, 5):
+    for bad in (-1, 6):
         with pytest.raises(BDError):
             e.set_walk(bad)
     for bad in (-1, 3):
