@@ -422,6 +422,20 @@ __global__ void __launch_bounds__(kBlk)
                      draws, step, total_lanes, nocup, ins, flags, do_bin, lm);
 }
 
+// Inclusive prefix sum over the 64 lanes of a wavefront: within each row of
+// 16 lanes by shifts of 1, 2, 4 and 8 lanes (lanes without a source add 0),
+// then the last lane of row 0 (and of row 2) added to row 1 (and row 3), and
+// the last lane of row 1 to rows 2 and 3.
+__device__ inline int wave_prefix_sum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
 // Row packing.  A wavefront packs rpw = 64 / n rows (x, y) of a grid with n
 // <= 64 cells per row (lane j n + z holds cell z of its j-th row), or one
 // row in chunks of 64 cells when n > 64.  Each lane reads its cell's
@@ -493,15 +507,10 @@ __device__ __forceinline__ void k_cells_block(
       }
     }
     const int m = occ < ntags ? occ : ntags;
-    int incl = m;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int v = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += v;
-    }
-    const int before = __shfl(incl, seg0 > 0 ? seg0 - 1 : 0, 64);
+    const int incl = wave_prefix_sum(m);
+    const int before = __builtin_amdgcn_ds_bpermute((seg0 > 0 ? seg0 - 1 : 0) << 2, incl);
     const int prior = seg0 > 0 ? before : 0;
-    const int row_sum = __shfl(incl, seg_last, 64) - prior;
+    const int row_sum = __builtin_amdgcn_ds_bpermute(seg_last << 2, incl) - prior;
     const int start = row_base + carry + (incl - prior) - m;
     if (live) {
       rrow[z] = start;
@@ -1466,20 +1475,6 @@ constexpr unsigned int kFlatSlotBits = 22;  // grid slot bits of a staged entry
 constexpr unsigned int kFlatSlotMask = (1u << kFlatSlotBits) - 1u;
 constexpr int kFlatMaxTags = 12;            // 5 cells x 12 slots fit the 6-bit length
 constexpr int kFlatBatch = 4;               // trips whose gathers are issued together
-
-// Inclusive prefix sum over the 64 lanes of a wavefront: within each row of
-// 16 lanes by shifts of 1, 2, 4 and 8 lanes (lanes without a source add 0),
-// then the last lane of row 0 (and of row 2) added to row 1 (and row 3), and
-// the last lane of row 1 to rows 2 and 3.
-__device__ inline int wave_prefix_sum(int v) {
-  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
-  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
-  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
-  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);  // row_shr:8
-  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
-  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
-  return v;
-}
 
 // Element idx of an array addressed by a 32-bit byte offset from its base
 // (one scalar base, one 32-bit vector offset per access); idx * sizeof(T)
