@@ -15,7 +15,7 @@ import orc
 from bd_aging_amd import abi
 from bd_aging_amd.abi import (BD_POT_WCA, BD_POT_WCA_MODIFIED, BD_POT_HARMONIC, BD_POT_WCA49_50,
                               BD_POT_WCA49_50_AO, BD_TEMP_SINE, BDError)
-from bd_aging_amd.engine import Engine, make_params, rng_stream, box_side
+from bd_aging_amd.engine import Engine, make_params, rng_stream, box_side, geometry
 from bd_aging_amd.init import relax_overlaps
 
 pytestmark = pytest.mark.gpu
@@ -432,17 +432,21 @@ def test_flat_walk_bit_identical(config1_positions):
 
 def test_flat_walk_batch_bit_identical():
     """The flat walk on an uneven batch whose 64-particle blocks straddle the
-    species boundary (big counts not multiples of 64) and whose replicas need
-    one or two stencil rings, switched on and off during the run, against
-    the slab walk: every replica's state equal after 300 steps."""
+    species boundary (big counts not multiples of 64), whose replicas need
+    one or two stencil rings, and one of which is too small for the fast
+    walks, switched on and off during the run, against the slab walk: every
+    replica's state equal after 300 steps."""
     specs = [(300, 340, 0.52, 1.5, 1), (701, 650, 0.55, 0.2, 2), (1000, 1000, 0.60, 0.05, 3),
-             (523, 480, 0.58, 0.7, 4)]
+             (523, 480, 0.58, 0.7, 4), (100, 100, 0.50, 0.4, 5)]
     params, starts = [], []
     for nb, ns, phi, T, seed in specs:
         side = box_side(nb, ns, 1.4, 1.0, phi)
         starts.append(relax_overlaps(nb, ns, side, seed=seed))
         params.append(make_params(nb, ns, side=side, T0=1.0, Tf=T, t0=0.0, tf=0.0,
                                   first_step=-5, seed=3000 + seed))
+    # the last replica's grids (4^3 big, 6^3 small cells) are too narrow for the
+    # fast walks: its workgroups take the per-lane walk inside the flat kernel
+    assert geometry(params[-1]).ncell_big < 5
     runs = []
     for mode in ("slabs", "flat"):
         e = Engine(params)
