@@ -15,7 +15,7 @@ batched ensemble's 2e5 steps, so the grid is capped at what the data cover.
 The network (bd_mlp_*: BF16 matrix-core kernels, FP32 accumulation) is an
 ensemble of 5 independently initialised members; a state point whose
 ensemble standard deviation exceeds `std_gate` at any lag falls back to
-plain BD, which stays the judge of the surrogate's error.
+plain BD, which stays the yardstick of the surrogate's error.
 """
 import ctypes as C
 import math
