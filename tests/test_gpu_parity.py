@@ -265,6 +265,13 @@ def test_pass_split_column_overflow(config1_positions):
     e.set_walk(abi.BD_WALK_LIST)  # 40 + listed candidates: the particle walks its cells
     e.compute_forces()
     assert all(np.array_equal(a, b) for a, b in zip(e.forces(), (fo, vo, uo)))
+    with pytest.raises(BDError):  # 24 slots per cell: beyond the flat walk's 6-bit range lengths
+        e.set_walk(abi.BD_WALK_FLAT)
+    assert e.walk() == abi.BD_WALK_LIST
+    e.autotune(2)  # passes over the flat shapes it cannot run
+    assert e.walk() != abi.BD_WALK_FLAT
+    e.compute_forces()
+    assert all(np.array_equal(a, b) for a, b in zip(e.forces(), (fo, vo, uo)))
     for walk in (abi.BD_WALK_SLABS, abi.BD_WALK_CHUNK2, abi.BD_WALK_CHUNK4):
         for depth in (16, 32):
             for split in (0, 1, 2):
