@@ -1746,14 +1746,15 @@ __global__ void __launch_bounds__(kBlk, 4)
 // One stencil pass (0: same species, 1: the other species, main.cu:576-585)
 // of replica-local particle i with a slab walk (WALK 1..3), on grids of at
 // least 2 nrange + 3 cells.  part selects the slabs di: -1 all, 0 = [-nrange,
-// 0], 1 = [1, nrange].  Returns false when the particle has no cell in that
-// grid.
+// 0], 1 = [1, nrange], 2 = [lo, hi].  Returns false when the particle has no
+// cell in that grid.
 // REC: record the partners in df only (see scan_chunks).
 template <int POT, int DEPTH, int WALK, bool REC>
 __device__ inline bool walk_pass(const RepDev& R, int i, int pass, double x, double y, double z,
                                  const int* __restrict__ rs, const double4* __restrict__ cpos,
                                  Defer& df, unsigned int* seg, double& fx, double& fy,
-                                 double& fz, double& v, double& u, int part = -1) {
+                                 double& fz, double& v, double& u, int part = -1, int lo = 0,
+                                 int hi = 0) {
   const int g = i < R.nb ? 0 : 1;
   const int gg = pass == 0 ? g : 1 - g;
   const int nr = pass == 0 ? R.nrange_same[g] : R.nrange_cross[g];
@@ -1767,8 +1768,8 @@ __device__ inline bool walk_pass(const RepDev& R, int i, int pass, double x, dou
   const int* rs_g = rs + R.rs_off[gg];
   const double4* cp_g = cpos + R.tag_off[gg];
   const unsigned int pb = pass == 0 ? 0u : kPassBit;
-  const int di0 = part == 1 ? 1 : -nr;
-  const int di1 = part == 0 ? 0 : nr;
+  const int di0 = part == 2 ? lo : (part == 1 ? 1 : -nr);
+  const int di1 = part == 2 ? hi : (part == 0 ? 0 : nr);
   if (WALK == 1) {
     if (nr == 1)
       scan_slabs<POT, 1, DEPTH, REC>(R, n, cs, sigma, rc2, self, x, y, z, hg, rs_g, cp_g, df, pb,
@@ -1936,9 +1937,38 @@ __global__ void __launch_bounds__(2 * kBlk)
                px, py, pz, mv);
 }
 
+// Wavefront w (1..3) of the four-wavefront split: its share of the slabs
+// after wavefront 0's same-species slabs [-nr0, 0], in visiting order (the
+// same-species slabs [1, nr0], then the cross-species slabs [-nr1, nr1]),
+// cut into three contiguous runs of nearly equal length (the first runs take
+// the remainder).  False when the particle has no cell in a grid it needs.
+template <int POT, int DEPTH, int WALK, bool REC>
+__device__ inline bool walk_quad_part(const RepDev& R, int i, int w, double x, double y,
+                                      double z, const int* __restrict__ rs,
+                                      const double4* __restrict__ cpos, Defer& df,
+                                      unsigned int* seg, double& fx, double& fy, double& fz,
+                                      double& v, double& u) {
+  const int g = i < R.nb ? 0 : 1;
+  const int nr0 = R.nrange_same[g], nr1 = R.nrange_cross[g];
+  const int rest = nr0 + 2 * nr1 + 1;
+  const int q = rest / 3, rem = rest % 3;
+  const int b = (w - 1) * q + min(w - 1, rem);  // run [b, e) of the remaining slabs
+  const int e = b + q + (w - 1 < rem ? 1 : 0);
+  bool ok = true;
+  if (b < nr0)  // remaining slab r < nr0 is same-species slab r + 1
+    ok = walk_pass<POT, DEPTH, WALK, REC>(R, i, 0, x, y, z, rs, cpos, df, seg, fx, fy, fz, v, u,
+                                          2, b + 1, min(e, nr0));
+  if (ok && e > nr0)  // remaining slab r >= nr0 is cross-species slab r - nr0 - nr1
+    ok = walk_pass<POT, DEPTH, WALK, REC>(R, i, 1, x, y, z, rs, cpos, df, seg, fx, fy, fz, v, u,
+                                          2, max(b, nr0) - nr0 - nr1, e - 1 - nr0 - nr1);
+  return ok;
+}
+
 // Stencil passes split over four wavefronts (no reference counterpart): for
-// the same 64 particles, wavefront w walks slab part (w & 1) of pass (w >> 1)
-// (walk_pass), so the walk of one particle runs on four lanes at once.
+// the same 64 particles, wavefront 0 walks the same-species slabs [-nr0, 0]
+// and wavefronts 1..3 share the remaining slabs (walk_quad_part: two each
+// for a big particle of the standard grids, whose cross-species pass has five
+// slabs), so the walk of one particle runs on four lanes at once.
 // Wavefronts 1..3 record their partners and evaluate the terms of the first
 // kTermCap of them into LDS; after the barrier wavefront 0 adds, in walking
 // order, its own terms, then each other wavefront's stored terms and the
@@ -1984,8 +2014,8 @@ __global__ void __launch_bounds__(4 * kBlk)
   if (w > 0) {
     int nw = -1;  // -1: wavefront 0 walks the part itself; -2: no cell in that grid
     if (live && R.fast) {
-      if (walk_pass<POT, DEPTH, WALK, true>(R, i, w >> 1, x, y, z, rs, cpos, df, seg, fx, fy,
-                                            fz, v, u, w & 1)) {
+      if (walk_quad_part<POT, DEPTH, WALK, true>(R, i, w, x, y, z, rs, cpos, df, seg, fx, fy,
+                                                 fz, v, u)) {
         nw = df.nd <= DEPTH ? df.nd : -1;
         double* tq = terms + (w - 1) * kTermCap * 5 * kBlk + lane;
         const int m = nw < kTermCap ? nw : kTermCap;
@@ -2015,7 +2045,7 @@ __global__ void __launch_bounds__(4 * kBlk)
       }
     }
     nrec[(w - 1) * kBlk + lane] = nw;
-    if (UPD && w == 3 && live) {  // the next move's normals (the lightest part's wavefront)
+    if (UPD && w == 3 && live) {  // the next move's normals (a part of at most two slabs)
       lane_normals(R, t.x, i, mv.ring, mv.lcgs, mv.draws, mv.total_lanes, nrm[lane],
                    nrm[kBlk + lane], nrm[2 * kBlk + lane]);
     }
@@ -2055,8 +2085,8 @@ __global__ void __launch_bounds__(4 * kBlk)
                          nw};
           drain_range<POT, kDrainBatch>(R, x, y, z, dk, kTermCap, nw, fx, fy, fz, v, u);
         }
-      } else if (walk_pass<POT, DEPTH, WALK, false>(R, i, k >> 1, x, y, z, rs, cpos, df, seg, fx,
-                                                    fy, fz, v, u, k & 1)) {
+      } else if (walk_quad_part<POT, DEPTH, WALK, false>(R, i, k, x, y, z, rs, cpos, df, seg,
+                                                         fx, fy, fz, v, u)) {
         drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
       }
     }
