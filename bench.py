@@ -44,8 +44,8 @@ RNG_BYTES = 112            # rannew64 ring and LCG bytes per particle-step (8 x 
 # profiles/k_forces_traffic_pmc.txt, scaled by a copy kernel of known size:
 # plain N=32768 with the four-wavefront chunked walk (forces only, and forces
 # with the fused update), batch with the slab walk (both at deferral depth 16)
-TRAFFIC_FORCES_PLAIN = 6.74e6
-TRAFFIC_FUSED_PLAIN = 1.484e7
+TRAFFIC_FORCES_PLAIN = 6.77e6
+TRAFFIC_FUSED_PLAIN = 1.487e7
 TRAFFIC_FORCES_BATCH = 1.489e8
 
 
