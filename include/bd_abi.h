@@ -267,9 +267,10 @@ int bd_get_list_skin(const bd_sys* s, double* skin);
  * walks both passes of a particle (main.cu:576-585) in one lane; 1 walks the
  * cross-species pass on a second wavefront at the same time, which only
  * records its partners for the first wavefront to evaluate after its own;
- * 2 walks the two halves of each pass on four wavefronts, the other three
- * evaluating the terms of their first partners into LDS for the first to
- * add in order.  More wavefronts for the same work, for systems too small to
+ * 2 walks a particle's stencil slabs on four wavefronts (the first takes
+ * the same-species slabs up to the centre, the other three the remaining
+ * slabs in nearly equal runs), the other three evaluating the terms of their
+ * first partners into LDS for the first to add in order.  More wavefronts for the same work, for systems too small to
  * fill the GPU.  Takes effect with a slab walk (BD_WALK_SLABS, _CHUNK2,
  * _CHUNK4) at a nonzero deferral depth.  With BD_WALK_FLAT, 1 and 2 give
  * each wavefront 32 and 16 particles instead of 64 (the 64 lanes still test
