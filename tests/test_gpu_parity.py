@@ -442,7 +442,8 @@ def test_flat_walk_batch_bit_identical():
     species boundary (big counts not multiples of 64), whose replicas need
     one or two stencil rings, and one of which is too small for the fast
     walks, switched on and off during the run, against the slab walk: every
-    replica's state equal after 300 steps."""
+    replica's state equal after 300 steps.  The same for the four-wavefront
+    split of the chunked walk."""
     specs = [(300, 340, 0.52, 1.5, 1), (701, 650, 0.55, 0.2, 2), (1000, 1000, 0.60, 0.05, 3),
              (523, 480, 0.58, 0.7, 4), (100, 100, 0.50, 0.4, 5)]
     params, starts = [], []
@@ -455,25 +456,35 @@ def test_flat_walk_batch_bit_identical():
     # fast walks: its workgroups take the per-lane walk inside the flat kernel
     assert geometry(params[-1]).ncell_big < 5
     runs = []
-    for mode in ("slabs", "flat"):
+    for mode in ("slabs", "flat", "split4"):
         e = Engine(params)
         for r, x in enumerate(starts):
             e.set_positions(x, r)
         if mode == "flat":
             e.set_walk(abi.BD_WALK_FLAT)
+        elif mode == "split4":  # four wavefronts per block, the slab shares of walk_quad_part
+            e.set_walk(abi.BD_WALK_CHUNK4)
+            e.set_defer_depth(16)
+            e.set_pass_split(2)
         e.step(100)
         if mode == "flat":
             e.set_walk(abi.BD_WALK_SLABS)
             e.step(37)
             e.set_walk(abi.BD_WALK_FLAT)
             e.step(163)
+        elif mode == "split4":
+            e.set_pass_split(0)
+            e.step(37)
+            e.set_pass_split(2)
+            e.step(163)
         else:
             e.step(200)
         runs.append([[e.positions(r), e.raw_positions(r), *e.forces(r)] for r in range(len(specs))])
         e.close()
-    for r in range(len(specs)):
-        for a, b in zip(runs[0][r], runs[1][r]):
-            assert np.array_equal(a, b), r
+    for other in (1, 2):
+        for r in range(len(specs)):
+            for a, b in zip(runs[0][r], runs[other][r]):
+                assert np.array_equal(a, b), (other, r)
 
 
 @pytest.fixture(scope="module")
