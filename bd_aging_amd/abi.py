@@ -122,6 +122,7 @@ SIGNATURES = {
     "bd_kernel_times_fused": (C.c_int, [_P, C.c_int32, _D]),
     "bd_stream_bandwidth": (C.c_int, [C.c_int64, C.c_int32, _D]),
     "bd_grid_barrier_cost": (C.c_int, [C.c_int32, C.c_int32, _D, _I]),
+    "bd_launch_floor": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _D]),
     "bd_checkpoint_save": (C.c_int, [_P, C.c_char_p]),
     "bd_checkpoint_load": (C.c_int, [_P, C.c_char_p]),
     "bd_get_rng_lane": (C.c_int, [_P, C.c_int32, C.c_int32, _U, _U, _U]),

@@ -3041,6 +3041,11 @@ __global__ void __launch_bounds__(kBlk) k_grid_barrier(int barriers, int* __rest
   if (blockIdx.x == 0 && threadIdx.x == 0) done[0] = barriers;
 }
 
+// A kernel that does next to nothing, for the launch-floor probe.
+__global__ void __launch_bounds__(1024) k_launch_floor(int* __restrict__ done) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) done[0] = 1;
+}
+
 // Stand-alone rannew64 lanes for the stream test.
 __global__ void k_rng_stream(uint64_t seed, int lanes, int ndraws, uint64_t* __restrict__ ring,
                              uint64_t* __restrict__ out) {
@@ -5041,6 +5046,36 @@ int bd_stream_bandwidth(int64_t bytes, int32_t reps, double* gbs) {
   if (e != hipSuccess)
     return fail(BD_E_HIP, std::string("bd_stream_bandwidth: ") + hipGetErrorString(e));
   *gbs = 2.0 * (double)(n * sizeof(double4)) * reps / (total_ms * 1e-3) / 1e9;
+  return BD_OK;
+}
+
+int bd_launch_floor(int32_t workgroups, int32_t threads, int32_t reps, double* us) {
+  if (workgroups < 1 || workgroups > (1 << 20) || threads < 64 || threads > 1024 ||
+      threads % 64 != 0 || reps < 1 || reps > 100000 || !us)
+    return fail(BD_E_ARG, "bad arguments");
+  int* done = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&done, sizeof(int));
+  if (e == hipSuccess) e = hipStreamCreate(&st);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double total_ms = 0.0;
+  for (int k = 0; k <= reps && e == hipSuccess; k++) {  // launch 0 warms up
+    hipExtLaunchKernelGGL(k_launch_floor, dim3(workgroups), dim3(threads), 0, st, e0, e1, 0,
+                          done);
+    e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (k > 0) total_ms += (double)t;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (done) (void)hipFree(done);
+  if (e != hipSuccess)
+    return fail(BD_E_HIP, std::string("bd_launch_floor: ") + hipGetErrorString(e));
+  *us = total_ms * 1e3 / reps;
   return BD_OK;
 }
 

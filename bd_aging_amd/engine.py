@@ -379,6 +379,14 @@ def grid_barrier_cost(workgroups, barriers=2000):
     return us.value, mx.value
 
 
+def launch_floor(workgroups, threads=64, reps=200):
+    """Mean microseconds of an (almost) empty kernel of `workgroups` x
+    `threads`, timed as the engine's kernel times are."""
+    v = C.c_double()
+    check(load().bd_launch_floor(int(workgroups), int(threads), int(reps), C.byref(v)))
+    return v.value
+
+
 class Contraction:
     """Force contraction (Contracting_Vector3D): add samples, solve, read."""
 

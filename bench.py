@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from bd_aging_amd.engine import (Contraction, Engine, make_params, box_side,  # noqa: E402
-                                 grid_barrier_cost, stream_bandwidth)
+                                 grid_barrier_cost, launch_floor, stream_bandwidth)
 from bd_aging_amd.init import relax_overlaps, relax_batch  # noqa: E402
 
 HBM_PEAK = 8.0e12          # B/s, MI355X
@@ -450,6 +450,10 @@ def main():
     achieved = f_bytes * N / (f_ms * 1e-3)
     copy_gbs = stream_bandwidth()
     log(f"[bench] device copy kernel: {copy_gbs:.0f} GB/s")
+    # an almost empty kernel, timed as the kernel times are: the per-launch
+    # part of every kernel time (one workgroup; the plain force kernel's shape)
+    floor = {"1x64": launch_floor(1, 64), "512x256": launch_floor(512, 256)}
+    log(f"[bench] launch floor us: {floor}")
     # the normal equations' multiply-adds: 3 axes x the (bins + 1)-column
     # augmented lower triangle x N particles
     cv3d_macs = 3 * (CV3D_BINS + 1) * (CV3D_BINS + 2) // 2 * N
@@ -479,6 +483,7 @@ def main():
         "kernel_ms": {"update": plain["kernel_ms"][0], "cells": plain["kernel_ms"][1],
                       "forces": plain["kernel_ms"][2],
                       "forces_with_update": plain["fused_ms"][1] if plain["fused_ms"] else None},
+        "launch_floor_us": floor,
         "step_hbm_frac": B_STEP * plain["rate"] / HBM_PEAK,
         "gder": plain["gder"],
         "persistent": plain["persistent"],

@@ -354,6 +354,13 @@ int bd_stream_bandwidth(int64_t bytes, int32_t reps, double* gbs);
  * cost that the persistent step kernel pays three times per step. */
 int bd_grid_barrier_cost(int32_t workgroups, int32_t barriers, double* us,
                          int32_t* max_workgroups);
+/* Launch-floor probe (no reference counterpart): a kernel of `workgroups`
+ * workgroups of `threads` threads (64..1024, a multiple of 64) that does next
+ * to nothing, launched `reps` times after a warm-up one, each timed by the
+ * start and end stamps the runtime puts on the kernel itself (as
+ * bd_kernel_times does); *us = mean microseconds per launch: the part of
+ * every kernel time that is not the kernel's work. */
+int bd_launch_floor(int32_t workgroups, int32_t threads, int32_t reps, double* us);
 
 /* ---- rannew64 lanes ---- */
 /* State of one lane of a replica: ring[250] (slot order) and the LCG word;

@@ -1058,3 +1058,15 @@ def test_grid_barrier_probe():
     with pytest.raises(BDError) as e:
         grid_barrier_cost(most + 1, 10)
     assert e.value.code == abi.BD_E_ARG
+
+
+def test_launch_floor_probe():
+    """The launch-floor probe times an almost empty kernel: a few microseconds,
+    well below the force kernels it is compared with, and checks its arguments."""
+    from bd_aging_amd.engine import launch_floor
+    one = launch_floor(1, 64, 50)
+    full = launch_floor(512, 256, 50)
+    assert 0.0 < one < 100.0 and 0.0 < full < 100.0
+    for bad in ((0, 64), (1, 32), (1, 100), (1, 2048)):
+        with pytest.raises(BDError):
+            launch_floor(*bad, 10)
