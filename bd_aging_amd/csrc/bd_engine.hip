@@ -1789,6 +1789,35 @@ __device__ inline bool walk_pass(const RepDev& R, int i, int pass, double x, dou
   return true;
 }
 
+// Slabs [b, e) of a particle's visiting order (the same-species slabs
+// -nr0..nr0, then the cross-species slabs -nr1..nr1) with walk_pass.  False
+// when the particle has no cell in a grid it needs.
+template <int POT, int DEPTH, int WALK, bool REC>
+__device__ inline bool walk_slab_run(const RepDev& R, int i, int b, int e, double x, double y,
+                                     double z, const int* __restrict__ rs,
+                                     const double4* __restrict__ cpos, Defer& df,
+                                     unsigned int* seg, double& fx, double& fy, double& fz,
+                                     double& v, double& u) {
+  const int g = i < R.nb ? 0 : 1;
+  const int nr0 = R.nrange_same[g], nr1 = R.nrange_cross[g];
+  const int s0 = 2 * nr0 + 1;
+  bool ok = true;
+  if (b < s0 && e > b)
+    ok = walk_pass<POT, DEPTH, WALK, REC>(R, i, 0, x, y, z, rs, cpos, df, seg, fx, fy, fz, v, u,
+                                          2, b - nr0, min(e, s0) - 1 - nr0);
+  if (ok && e > s0)
+    ok = walk_pass<POT, DEPTH, WALK, REC>(R, i, 1, x, y, z, rs, cpos, df, seg, fx, fy, fz, v, u,
+                                          2, max(b, s0) - s0 - nr1, e - 1 - s0 - nr1);
+  return ok;
+}
+
+// Slab counts of particle i's visiting order: same-species, all.
+__device__ inline void slab_counts(const RepDev& R, int i, int& s0, int& s) {
+  const int g = i < R.nb ? 0 : 1;
+  s0 = 2 * R.nrange_same[g] + 1;
+  s = s0 + 2 * R.nrange_cross[g] + 1;
+}
+
 // The five terms one recorded pair adds (pair_add's operations, split from
 // the accumulation): force components, virial term, energy term; qc is the
 // partner's slot, already loaded.
@@ -1848,16 +1877,17 @@ __device__ inline void drain_range(const RepDev& R, double x, double y, double z
 constexpr int kDrainBatch = 4;
 
 // Stencil passes split over two wavefronts (no reference counterpart): for
-// the same 64 particles, wavefront 0 walks the same-species stencil and
-// evaluates its pair terms, while wavefront 1 walks the cross-species stencil
-// and only records its partners in LDS; after the barrier wavefront 0
-// evaluates those records after its own, so every pair term is added in the
-// single-wavefront kernel's order (bit-identical).  A particle whose
-// cross-species partners overflow the DEPTH-entry column has that pass walked
-// again by wavefront 0.  Twice the wavefronts of k_forces for the same work,
+// the same 64 particles, wavefront 0 walks the first half of the slabs in
+// visiting order (the same-species stencil, and for a big particle of the
+// standard grids the first cross-species slab too) and evaluates its pair
+// terms, while wavefront 1 walks the rest and only records its partners in
+// LDS; after the barrier wavefront 0 evaluates those records after its own,
+// so every pair term is added in the single-wavefront kernel's order
+// (bit-identical).  A particle whose second-half partners overflow the
+// DEPTH-entry column has that half walked again by wavefront 0.  Twice the wavefronts of k_forces for the same work,
 // for systems too small to fill the GPU with one lane per particle.
 template <int POT, int DEPTH, int WALK, bool UPD>
-__global__ void __launch_bounds__(2 * kBlk)
+__global__ void __launch_bounds__(2 * kBlk, (WALK == 1 && DEPTH <= 16) ? 4 : 1)
     k_forces_pair(const RepDev* __restrict__ reps, const int4* __restrict__ tbl,
                   double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
                   const int* __restrict__ rs, const double4* __restrict__ cpos,
@@ -1888,11 +1918,14 @@ __global__ void __launch_bounds__(2 * kBlk)
            dbuf + w * DEPTH * kBlk + lane, 0};
   unsigned int* seg = sbuf + w * 2 * kSlabSegs * kBlk + lane;
   bool ok = true;
+  int s0, ns;
+  slab_counts(R, i, s0, ns);
+  const int half = ns / 2;  // wavefront 0: slabs [0, half) of the visiting order
   if (w == 1) {
-    int n1 = -1;  // -1: wavefront 0 walks the pass itself; -2: no cell in that grid
+    int n1 = -1;  // -1: wavefront 0 walks the run itself; -2: no cell in a grid
     if (live && R.fast) {
-      if (walk_pass<POT, DEPTH, WALK, true>(R, i, 1, x, y, z, rs, cpos, df, seg, fx, fy, fz, v,
-                                            u))
+      if (walk_slab_run<POT, DEPTH, WALK, true>(R, i, half, ns, x, y, z, rs, cpos, df, seg, fx,
+                                                fy, fz, v, u))
         n1 = df.nd <= DEPTH ? df.nd : -1;
       else
         n1 = -2;
@@ -1904,8 +1937,8 @@ __global__ void __launch_bounds__(2 * kBlk)
     }
   } else if (live) {
     if (R.fast)
-      ok = walk_pass<POT, DEPTH, WALK, false>(R, i, 0, x, y, z, rs, cpos, df, seg, fx, fy, fz, v,
-                                              u);
+      ok = walk_slab_run<POT, DEPTH, WALK, false>(R, i, 0, half, x, y, z, rs, cpos, df, seg, fx,
+                                                  fy, fz, v, u);
     else
       forces_global<POT, false>(R, i, x, y, z, rs, cpos, fx, fy, fz, v, u);
   }
@@ -1921,8 +1954,8 @@ __global__ void __launch_bounds__(2 * kBlk)
       if (n1 >= 0) {
         Defer d1{df.cp0, df.cp1, df.sigma0, df.sigma1, dbuf + DEPTH * kBlk + lane, n1};
         drain_range<POT, kDrainBatch>(R, x, y, z, d1, 0, n1, fx, fy, fz, v, u);
-      } else if (walk_pass<POT, DEPTH, WALK, false>(R, i, 1, x, y, z, rs, cpos, df, seg, fx, fy,
-                                                    fz, v, u)) {
+      } else if (walk_slab_run<POT, DEPTH, WALK, false>(R, i, half, ns, x, y, z, rs, cpos, df,
+                                                        seg, fx, fy, fz, v, u)) {
         drain<POT>(R, x, y, z, df, fx, fy, fz, v, u);
       }
     }
@@ -1948,20 +1981,15 @@ __device__ inline bool walk_quad_part(const RepDev& R, int i, int w, double x, d
                                       const double4* __restrict__ cpos, Defer& df,
                                       unsigned int* seg, double& fx, double& fy, double& fz,
                                       double& v, double& u) {
-  const int g = i < R.nb ? 0 : 1;
-  const int nr0 = R.nrange_same[g], nr1 = R.nrange_cross[g];
-  const int rest = nr0 + 2 * nr1 + 1;
+  int s0, s;
+  slab_counts(R, i, s0, s);
+  const int first = (s0 + 1) / 2;  // wavefront 0's slabs [-nr0, 0]
+  const int rest = s - first;
   const int q = rest / 3, rem = rest % 3;
-  const int b = (w - 1) * q + min(w - 1, rem);  // run [b, e) of the remaining slabs
+  const int b = first + (w - 1) * q + min(w - 1, rem);
   const int e = b + q + (w - 1 < rem ? 1 : 0);
-  bool ok = true;
-  if (b < nr0)  // remaining slab r < nr0 is same-species slab r + 1
-    ok = walk_pass<POT, DEPTH, WALK, REC>(R, i, 0, x, y, z, rs, cpos, df, seg, fx, fy, fz, v, u,
-                                          2, b + 1, min(e, nr0));
-  if (ok && e > nr0)  // remaining slab r >= nr0 is cross-species slab r - nr0 - nr1
-    ok = walk_pass<POT, DEPTH, WALK, REC>(R, i, 1, x, y, z, rs, cpos, df, seg, fx, fy, fz, v, u,
-                                          2, max(b, nr0) - nr0 - nr1, e - 1 - nr0 - nr1);
-  return ok;
+  return walk_slab_run<POT, DEPTH, WALK, REC>(R, i, b, e, x, y, z, rs, cpos, df, seg, fx, fy, fz,
+                                              v, u);
 }
 
 // Stencil passes split over four wavefronts (no reference counterpart): for

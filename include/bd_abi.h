@@ -265,8 +265,9 @@ int bd_get_list_skin(const bd_sys* s, double* skin);
 
 /* Stencil passes over wavefronts (no reference counterpart): 0 (default)
  * walks both passes of a particle (main.cu:576-585) in one lane; 1 walks the
- * cross-species pass on a second wavefront at the same time, which only
- * records its partners for the first wavefront to evaluate after its own;
+ * second half of the particle's stencil slabs (in visiting order) on a second
+ * wavefront at the same time, which only records its partners for the first
+ * wavefront to evaluate after its own;
  * 2 walks a particle's stencil slabs on four wavefronts (the first takes
  * the same-species slabs up to the centre, the other three the remaining
  * slabs in nearly equal runs), the other three evaluating the terms of their
