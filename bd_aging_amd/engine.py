@@ -258,8 +258,9 @@ class Engine:
 
     def set_pass_split(self, split):
         """Wavefronts per particle block (slab walks with deferral): 0 one,
-        1 two (each half of the slabs in visiting order), 2 four (the first the same-species slabs
-        up to the centre, the others the remaining slabs in nearly equal runs)."""
+        1 two (each half of the slabs in visiting order), 2 four (the first
+        the same-species slabs up to the centre, the others the remaining
+        slabs in nearly equal runs)."""
         check(self.lib.bd_set_pass_split(self.h, int(split)))
 
     def pass_split(self):
